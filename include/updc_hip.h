@@ -1,0 +1,197 @@
+/* updc_hip.h -- C ABI of the MI355X (gfx950) UPDC v17 reconstruction library.
+ *
+ * Plain C types only (pointers, sizes, ints, doubles): the library can be
+ * bound from ctypes, cgo, JNI or a C program.  Every entry point returns a
+ * status code (UPDC_OK on success) instead of calling exit(); the message of
+ * the last failure on the calling thread is available from updc_last_error().
+ * Host pointers are read during the call only (the library copies what it
+ * keeps).  One context is driven by one host thread at a time.
+ *
+ * Interfaces replaced (reference files cited as path:line):
+ *   updc_all            <- UPDC_ALL(...)           updc/c_core/updc_core.c:1289-1293
+ *                          and the updc_jwst_c argv  updc/c_core/updc_core.c:1566-1573
+ *   updc_self_test      <- self_test()             updc/c_core/updc_core.c:1545-1564
+ *   updc_polygon_area   <- updc_polygon_area()     updc/c_core/overlap_geometry.h:15
+ *   updc_overlap_area   <- updc_overlap_area()     updc/c_core/overlap_geometry.h:16
+ *   updc_ctx_* / updc_add_exposure* / updc_prepare / updc_iterate
+ *                       <- the in-process stages of UPDC_ALL: manifest rows
+ *                          (:426-461), LinkMap build/open (:530-663), weights
+ *                          (:378-424), PSF spectra (:777-876), initial drizzle
+ *                          (:878-983), denominator (:985-1086) and the
+ *                          iteration loop (:1410-1479).
+ */
+#ifndef UPDC_HIP_H
+#define UPDC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define UPDC_OK 0
+#define UPDC_ERR_ARG 1      /* invalid argument or dimensions                  */
+#define UPDC_ERR_HIP 2      /* HIP runtime / device failure                    */
+#define UPDC_ERR_IO 3       /* file missing, short, or with trailing bytes     */
+#define UPDC_ERR_NUMERIC 4  /* a numeric guard of the reference fired          */
+#define UPDC_ERR_STATE 5    /* call out of order (e.g. iterate before prepare) */
+#define UPDC_ERR_NOMEM 6    /* device or host allocation failed                */
+#define UPDC_ERR_FORMAT 7   /* invalid / obsolete LinkMap or cache file        */
+
+/* ---- precision tiers ----------------------------------------------------- */
+#define UPDC_PREC_F32 0   /* the reference's float32 arithmetic (docs/PRECISION.md) */
+#define UPDC_PREC_F64 1   /* plain FP64 form                                        */
+#define UPDC_PREC_MIXED 2 /* FP32 FFTs, PSF and S_k products; FP64 spectrum sum over
+                             exposures, denominator, raw_mu test and ratio         */
+
+/* ---- context flags ------------------------------------------------------- */
+#define UPDC_FLAG_GRAPH 1u      /* capture each iteration as a hipGraph and replay */
+#define UPDC_FLAG_QUIET 2u      /* no informational stdout                          */
+#define UPDC_FLAG_PER_EXPOSURE 4u /* one launch per exposure and stage (no batching) */
+
+/* ---- LinkMap ABI (updc/cache_abi.py:9-12, updc_core.c:44-54) -------------- */
+#define UPDC_LINKMAP_MAGIC 0x324D4C55u
+#define UPDC_LINKMAP_VERSION 2u
+#define UPDC_LINKMAP_ABS32 1u
+#define UPDC_LINKMAP_DELTA16 2u
+#define UPDC_LINKMAP_HEADER_BYTES 64u
+
+/* ---- drop-in solver -------------------------------------------------------- */
+/* Same 18 parameters, file contract and caches as UPDC_ALL
+ * (updc_core.c:1289-1543); returns a status instead of exit(EXIT_FAILURE).
+ * The worker-count parameters are accepted and validated as the reference
+ * does (>= 1); the GPU path has no thread pools.  plan_measure, autotune,
+ * retune_fft and wisdom_path steer the GPU FFT shape autotuner and its cache. */
+int updc_all(const char *manifest, int width, int height, int input_width, int input_height, int exposures,
+             int transpose_psf, int iterations, const char *iter_prefix, const char *weight_output,
+             int fft_worker_count, int exposure_worker_count, int preprocess_worker_count, int plan_measure,
+             int autotune, int retune_fft, const char *wisdom_path, int save_every);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *updc_last_error(void);
+
+/* updc_core.c:1545-1564: unit-square area, half-square overlap, LinkMap header
+ * size.  Returns 0 when every check passes (and prints the reference's OK line
+ * when verbose != 0). */
+int updc_self_test(int verbose);
+
+/* Host-side geometry with the reference's precision policy (f64 clipping,
+ * f32 local-origin shoelace with fmaf): overlap.c:72-116.  xy holds count
+ * (x, y) pairs; quad_xy holds 4 pairs. */
+float updc_polygon_area(const double *xy, int count);
+float updc_overlap_area(const double *quad_xy, int x, int y);
+
+/* ---- in-memory context API --------------------------------------------- */
+typedef struct updc_ctx updc_ctx;
+
+/* Per-(iteration, exposure) FreezeMu record: one mufreeze CSV row
+ * (updc_core.c:1425-1431). */
+typedef struct {
+    int32_t iteration;
+    int32_t exposure; /* 1-based, as in the CSV */
+    uint64_t n_new_frozen;
+    uint64_t n_total_frozen;
+    double min_raw_mu;
+} updc_freeze_row;
+
+/* Per-iteration summary (the reference's per-iteration stdout line,
+ * updc_core.c:1473-1478), with GPU stage times in milliseconds. */
+typedef struct {
+    int32_t iteration;
+    int32_t pad;
+    double signed_sum;
+    uint64_t n_negative;
+    uint64_t n_new_frozen;
+    uint64_t n_total_frozen;
+    double min_raw_mu;
+    double ms_total;
+} updc_iter_info;
+
+/* fft_width/fft_height of 0 pick the shape automatically (the smallest even
+ * 7-smooth sizes >= target + max PSF - 1, or the autotuner's choice when
+ * UPDC_FLAG is set through updc_ctx_set_autotune).  precision is one of
+ * UPDC_PREC_*; device is the HIP device ordinal. */
+int updc_ctx_create(updc_ctx **out, int width, int height, int fft_width, int fft_height, int precision,
+                    unsigned flags, int device);
+int updc_ctx_destroy(updc_ctx *ctx);
+
+/* Drizzle worker topology of the initial estimate (updc_core.c:917-973):
+ * 1 = the serial summation order, n > 1 = per-worker partial sums combined in
+ * worker order.  Default 1. */
+int updc_ctx_set_preprocess_workers(updc_ctx *ctx, int workers);
+
+/* One exposure (one manifest row, updc_core.c:426-461).  sci/wht: ih*iw raw
+ * float32 planes (not yet normalised).  psf: ph*pw float32 raster (stored
+ * transposed when psf_transposed, updc_core.c:814-820).  The LinkMap is
+ * either a complete ULM2 image (linkmap/linkmap_bytes, validated as
+ * updc_core.c:485-496 does) or built on the GPU from a vertex lattice of
+ * (ih+1)*(iw+1) (x,y) float64 pairs in target pixels (updc_core.c:530-609).
+ * Exposures may differ in iw/ih and PSF size. */
+int updc_add_exposure(updc_ctx *ctx, const float *sci, const float *wht, int input_width, int input_height,
+                      const void *linkmap, size_t linkmap_bytes, const float *psf, int psf_width, int psf_height,
+                      int psf_transposed, double background, double pedestal);
+int updc_add_exposure_vertices(updc_ctx *ctx, const float *sci, const float *wht, int input_width,
+                               int input_height, const double *vertices, const float *psf, int psf_width,
+                               int psf_height, int psf_transposed, double background, double pedestal);
+
+/* Bytes of exposure k's LinkMap (valid after add) and a copy of its ULM2
+ * image (for the persistent .links cache, updc_core.c:574-601). */
+int updc_linkmap_size(updc_ctx *ctx, int k, size_t *bytes);
+int updc_get_linkmap(updc_ctx *ctx, int k, void *out, size_t capacity);
+
+/* Setup: weight scale and normalisation, CSC transpose, PSF spectra, initial
+ * drizzle, denominator and its maximum (updc_core.c:1367-1398). */
+int updc_prepare(updc_ctx *ctx);
+/* Run n iterations.  rows (optional) receives n*N freeze records, info
+ * (optional) n per-iteration summaries. */
+int updc_iterate(updc_ctx *ctx, int n, updc_freeze_row *rows, updc_iter_info *info);
+
+/* Results and setup products (host copies; f32 outputs are what the reference
+ * writes, f64 variants return the working precision). */
+int updc_get_estimate(updc_ctx *ctx, float *out);
+int updc_get_estimate_f64(updc_ctx *ctx, double *out);
+int updc_set_estimate_f64(updc_ctx *ctx, const double *in);
+int updc_get_wht(updc_ctx *ctx, float *out);                /* coverage * weight_scale */
+int updc_get_denominator_f64(updc_ctx *ctx, double *out);   /* H x W                    */
+int updc_get_initial_f64(updc_ctx *ctx, double *estimate, double *coverage);
+int updc_get_psf_spectrum_f64(updc_ctx *ctx, int k, double *out); /* FH x (FW/2+1) complex, interleaved */
+int updc_get_scalars(updc_ctx *ctx, double *weight_scale, double *max_sensitivity, int *fft_width,
+                     int *fft_height, uint64_t *masked_pixels);
+int updc_get_frozen(updc_ctx *ctx, int k, uint8_t *out);
+
+/* Stage entry points (parity tests, autograd).  All host buffers, f64 at the
+ * boundary, computed in the context's precision on the GPU. */
+/* blur_k = crop(C2R(X^ . P^_k)) / nf for every exposure: out is N*H*W. */
+int updc_stage_forward(updc_ctx *ctx, const double *x, double *blur_out);
+/* Noise-free model H_k x = S_k P_k x for every pixel of every exposure
+ * (flattened, sum of nin), the forward gather of updc_core.c:1154-1158. */
+int updc_stage_model(updc_ctx *ctx, const double *x, double *pred_out);
+/* Detector stage of exposure k given blur_k (H*W): FreezeMu update of the
+ * context's masks, wr (nin) and the transposed-gather scatter (H*W).
+ * stats = {n_new, n_total, min_raw_mu}. */
+int updc_stage_detector(updc_ctx *ctx, int k, const double *blur, double *wr_out, double *scatter_out,
+                        double *stats);
+/* Adjoint half: C = crop(C2R(sum_k conj(P^_k) . R2C(pad(s_k)))) / nf where
+ * s_k = S_k^T v_k, v is the flattened per-pixel vector (sum of nin). */
+int updc_stage_adjoint(updc_ctx *ctx, const double *v, double *out);
+/* S_k^T v for one exposure (H*W), via the transposed gather. */
+int updc_stage_scatter(updc_ctx *ctx, int k, const double *v, double *out);
+/* Zero every FreezeMu mask. */
+int updc_reset_frozen(updc_ctx *ctx);
+
+/* 2-D FFT self-check (the GPU transforms on their own): image FH x FW real ->
+ * half spectrum FH x (FW/2+1) interleaved complex, and back (unnormalised). */
+int updc_fft_r2c_f64(updc_ctx *ctx, const double *image, double *spectrum);
+int updc_fft_c2r_f64(updc_ctx *ctx, const double *spectrum, double *image);
+
+/* Average device time (ms) per launch of each kernel class over the last
+ * updc_iterate call, measured with HIP events on the library's stream:
+ * names[i] / ms[i] / launches[i] for i < *count (capacity in, count out). */
+int updc_get_stage_times(updc_ctx *ctx, char names[][32], double *ms, int *launches, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UPDC_HIP_H */

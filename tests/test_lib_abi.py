@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every entry point include/updc_hip.h
+declares (no compute call: runs without a GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions() -> list[str]:
+    text = (ROOT / "include" / "updc_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(updc_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("updc_all", "updc_last_error", "updc_ctx_create", "updc_iterate", "updc_prepare"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from updc_amd import _lib
+    lib = _lib.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # the ctypes table covers the header exactly
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+def test_self_test_host_geometry():
+    """updc_self_test only exercises host code (updc_core.c:1545-1564)."""
+    from updc_amd import _lib
+    assert _lib.load().updc_self_test(0) == 0
+
+
+def test_dropin_executable_self_test():
+    import subprocess
+    from updc_amd import _lib
+    r = subprocess.run([str(_lib.EXE_PATH), "--self-test"], capture_output=True, text=True)
+    assert r.returncode == 0 and "UPDC C core self-test: OK" in r.stdout
+    bad = subprocess.run([str(_lib.EXE_PATH), "a", "b"], capture_output=True, text=True)
+    assert bad.returncode == 1 and bad.stderr.startswith("UPDC_ALL: usage")

@@ -1,0 +1,238 @@
+// Shared device/host helpers for the MI355X (gfx950) UPDC library.
+//
+// Everything here is written for CDNA4 directly: 64-lane wavefronts, LDS
+// resident transforms, no CUDA shims.  The whole library is compiled with
+// -ffp-contract=off so that only the explicit fma()/fmaf() calls fuse, which
+// is what keeps the detector and geometry kernels bit-identical to the
+// reference's float32 C core (built -std=c11, updc/c_core/Makefile:3).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define UPDC_HD __host__ __device__ __forceinline__
+
+// ---------------------------------------------------------------- complex
+template <class T> struct Cx { T re, im; };
+
+template <class T> UPDC_HD Cx<T> cx(T r, T i) { Cx<T> c; c.re = r; c.im = i; return c; }
+template <class T> UPDC_HD Cx<T> operator+(Cx<T> a, Cx<T> b) { return cx<T>(a.re + b.re, a.im + b.im); }
+template <class T> UPDC_HD Cx<T> operator-(Cx<T> a, Cx<T> b) { return cx<T>(a.re - b.re, a.im - b.im); }
+template <class T> UPDC_HD Cx<T> cmul(Cx<T> a, Cx<T> b) { return cx<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+// a * conj(b)
+template <class T> UPDC_HD Cx<T> cmulc(Cx<T> a, Cx<T> b) { return cx<T>(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im); }
+template <class T> UPDC_HD Cx<T> cscale(Cx<T> a, T s) { return cx<T>(a.re * s, a.im * s); }
+template <class T> UPDC_HD Cx<T> cconj(Cx<T> a) { return cx<T>(a.re, -a.im); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, class T> UPDC_HD Cx<T> rot90(Cx<T> a) { return INV ? cx<T>(-a.im, a.re) : cx<T>(a.im, -a.re); }
+
+// ---------------------------------------------------------------- math consts
+// cos/sin(2*pi*m/P) for the odd prime radices, evaluated to 22 digits from
+// their series (plain mathematics, no library internals).
+template <int P> struct OddRoots;
+template <> struct OddRoots<3> {
+    static constexpr double c[2] = {1.0, -0.5};
+    static constexpr double s[2] = {0.0, 0.8660254037844386467637};
+};
+template <> struct OddRoots<5> {
+    static constexpr double c[3] = {1.0, 0.3090169943749474241022, -0.8090169943749474241023};
+    static constexpr double s[3] = {0.0, 0.9510565162951535721164, 0.5877852522924731291687};
+};
+template <> struct OddRoots<7> {
+    static constexpr double c[4] = {1.0, 0.6234898018587335305250, -0.2225209339563144042889, -0.9009688679024191262361};
+    static constexpr double s[4] = {0.0, 0.7818314824680298087084, 0.9749279121818236070181, 0.4338837391175581204757};
+};
+#define UPDC_SQRT1_2 0.7071067811865475244008
+
+// ---------------------------------------------------------------- butterflies
+// In-place DFT of length R on v[0..R): y_j = sum_q v_q exp(-+2 pi i j q / R)
+// (minus for forward, plus for inverse), unnormalised.
+template <bool INV, class T> __device__ __forceinline__ void bfly2(Cx<T> *v) {
+    Cx<T> a = v[0], b = v[1];
+    v[0] = a + b; v[1] = a - b;
+}
+template <bool INV, class T> __device__ __forceinline__ void bfly4(Cx<T> *v) {
+    Cx<T> a0 = v[0] + v[2], a1 = v[0] - v[2];
+    Cx<T> b0 = v[1] + v[3], b1 = rot90<INV>(v[1] - v[3]);
+    v[0] = a0 + b0; v[2] = a0 - b0;
+    v[1] = a1 + b1; v[3] = a1 - b1;
+}
+template <bool INV, class T> __device__ __forceinline__ void bfly8(Cx<T> *v) {
+    // two radix-4 on even/odd samples, then radix-2 with w8 twiddles
+    Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    bfly4<INV>(e);
+    bfly4<INV>(o);
+    const T h = (T)UPDC_SQRT1_2;
+    // o1 *= w8^1, o2 *= w8^2 = -+i, o3 *= w8^3
+    Cx<T> t1 = o[1], t3 = o[3];
+    if (!INV) {
+        o[1] = cx<T>((t1.re + t1.im) * h, (t1.im - t1.re) * h);
+        o[3] = cx<T>((t3.im - t3.re) * h, -(t3.re + t3.im) * h);
+    } else {
+        o[1] = cx<T>((t1.re - t1.im) * h, (t1.im + t1.re) * h);
+        o[3] = cx<T>(-(t3.re + t3.im) * h, (t3.re - t3.im) * h);
+    }
+    o[2] = rot90<INV>(o[2]);
+    for (int j = 0; j < 4; ++j) { v[j] = e[j] + o[j]; v[j + 4] = e[j] - o[j]; }
+}
+template <int P, bool INV, class T> __device__ __forceinline__ void bfly_odd(Cx<T> *v) {
+    constexpr int H = (P - 1) / 2;
+    Cx<T> a[H + 1], b[H + 1];
+    Cx<T> sum = v[0];
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+        a[m] = v[m] + v[P - m];
+        b[m] = v[m] - v[P - m];
+        sum = sum + a[m];
+    }
+    Cx<T> out[P];
+    out[0] = sum;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        Cx<T> re = v[0], im = cx<T>((T)0, (T)0);
+#pragma unroll
+        for (int m = 1; m <= H; ++m) {
+            const int jm = (j * m) % P;
+            const T c = (T)OddRoots<P>::c[jm <= H ? jm : P - jm];
+            const T s = (jm <= H) ? (T)OddRoots<P>::s[jm] : -(T)OddRoots<P>::s[P - jm];
+            re = re + cscale(a[m], c);
+            im = im + cscale(b[m], s);
+        }
+        // forward: y_j = re - i*im, y_{P-j} = re + i*im
+        Cx<T> iim = cx<T>(-im.im, im.re);
+        if (!INV) { out[j] = re - iim; out[P - j] = re + iim; }
+        else { out[j] = re + iim; out[P - j] = re - iim; }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) v[j] = out[j];
+}
+template <int R, bool INV, class T> __device__ __forceinline__ void bfly(Cx<T> *v) {
+    if constexpr (R == 2) bfly2<INV>(v);
+    else if constexpr (R == 4) bfly4<INV>(v);
+    else if constexpr (R == 8) bfly8<INV>(v);
+    else bfly_odd<R, INV>(v);
+}
+
+// ---------------------------------------------------------------- FFT plan
+#define UPDC_MAX_STAGES 16
+// One transform length, split into Stockham radix stages.  tw points at the
+// device table tw[m] = exp(-2 pi i m / L), m in [0, L), of the FFT's real type.
+// For stage s, ns[s] is the product of the earlier radices and mg[s] =
+// ceil(2^32 / ns[s]), so i / ns[s] = umulhi(i, mg[s]) exactly for i, ns < 2^16.
+struct FftLen {
+    int L;
+    int nst;
+    int rad[UPDC_MAX_STAGES];
+    int ns[UPDC_MAX_STAGES];
+    unsigned int mg[UPDC_MAX_STAGES];
+    const void *tw;
+};
+
+// One Stockham (decimation in time, self-sorting) radix-R stage over
+// `1 << lg` interleaved transforms held in LDS: element n of line c lives at
+// buf[(n << lg) + c].  Butterfly b handles line c = b & (lines-1), index
+// i = b >> lg, reads the R values i + q L/R, applies the twiddles
+// exp(-+2 pi i q k / (Ns R)) with k = i mod Ns, and writes the outputs to
+// (i / Ns) Ns R + k + q Ns.  All reads finish before the barrier, so the stage
+// runs in place in one LDS buffer.  VMAX bounds the complex values one thread
+// holds per stage; the host checks R * ceil(nb / NT) <= VMAX.
+template <int R, bool INV, class T, int NT, int VMAX>
+__device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int Ns, unsigned int mg,
+                                          const Cx<T> *__restrict__ tw) {
+    constexpr int NB = VMAX / R;
+    static_assert(NB >= 1, "VMAX too small for radix");
+    const int lm = (1 << lg) - 1;
+    const int nb = (L / R) << lg;
+    const int LR = L / R;
+    const int tstride = L / (Ns * R);
+    Cx<T> v[NB][R];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        const int b = tid + r * NT;
+        if (b < nb) {
+            const int c = b & lm, i = b >> lg;
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[r][q] = buf[((i + q * LR) << lg) + c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        const int b = tid + r * NT;
+        if (b < nb) {
+            const int c = b & lm, i = b >> lg;
+            const int hi = (int)__umulhi((unsigned int)i, mg);
+            const int k = i - hi * Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    Cx<T> w = tw[q * k * tstride];
+                    if (INV) w.im = -w.im;
+                    v[r][q] = cmul(v[r][q], w);
+                }
+            }
+            bfly<R, INV>(v[r]);
+            const int j = hi * Ns * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[((j + q * Ns) << lg) + c] = v[r][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform over the LDS buffer (natural order in, natural order out).
+template <bool INV, class T, int NT, int VMAX>
+__device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg) {
+    const Cx<T> *tw = (const Cx<T> *)p.tw;
+    for (int s = 0; s < p.nst; ++s) {
+        const int R = p.rad[s], Ns = p.ns[s];
+        const unsigned int mg = p.mg[s];
+        switch (R) {
+            case 2: fft_stage<2, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
+            case 3: fft_stage<3, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
+            case 4: fft_stage<4, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
+            case 5: fft_stage<5, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
+            case 7: fft_stage<7, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
+            default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- XCD-aware tiles
+// Workgroups are dealt round-robin to the 8 XCDs.  Remap so each XCD walks
+// a contiguous range of tiles: neighbouring column tiles then share L2 lines
+// of the same XCD.  Returns -1 for padding workgroups.
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
+    const int per = (ntiles + 7) / 8;
+    const int t = (bid % 8) * per + bid / 8;
+    return t < ntiles ? t : -1;
+}
+
+// ---------------------------------------------------------------- ordered keys
+// Monotone float/double -> unsigned maps for atomicMin of signed values.
+__device__ __forceinline__ unsigned int fkey(float f) {
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ unsigned long long dkey(double d) {
+    unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+UPDC_HD float fkey_inv(unsigned int u) {
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+UPDC_HD double dkey_inv(unsigned long long u) {
+    u = (u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u;
+    double d;
+    __builtin_memcpy(&d, &u, 8);
+    return d;
+}
+
+// fused multiply-add in the storage type (fmaf for float, fma for double)
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }

@@ -1,0 +1,357 @@
+// updc_all: the drop-in replacement of the reference's UPDC_ALL
+// (updc/c_core/updc_core.c:1289-1543) over the context API.
+//
+// File contract kept as-is (updc_core.c:298-315, 426-461, 463-663, 1404-1508):
+//   manifest.txt rows  science \t weight \t psf \t psf_w \t psf_h \t vertices \t background \t pedestal
+//   raw planes         little-endian float32, exact size (no trailing bytes)
+//   <vertices>.links   ULM2 LinkMap, reused when UPDC_PERSIST_CACHE is set and the
+//                      header/size validate, rebuilt (on the GPU) otherwise
+//   <prefix>_%02d.bin  saved iterations (save_every rule, last always saved)
+//   <wht_path>         coverage * weight_scale, float32
+//   <prefix>_mufreeze.csv, <prefix>_timing.csv
+// Errors return a status and a message instead of exiting.
+#include "updc_hip.h"
+
+#include <cerrno>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+namespace {
+
+struct DErr : std::runtime_error {
+    int code;
+    DErr(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Row {
+    std::string sci, wht, psf, vert, links;
+    int pw = 0, ph = 0;
+    float bg_f = 0, ped_f = 0;
+    double bg = 0, ped = 0;
+};
+
+// updc_core.c:426-461
+std::vector<Row> parse_manifest(const char *path, int n) {
+    FILE *f = std::fopen(path, "r");
+    if (!f) throw DErr(UPDC_ERR_IO, "cannot open manifest");
+    std::vector<Row> rows;
+    char line[8192];
+    for (int k = 0; k < n; ++k) {
+        if (!std::fgets(line, sizeof line, f)) {
+            std::fclose(f);
+            throw DErr(UPDC_ERR_ARG, "manifest has fewer rows than expected");
+        }
+        char *save = nullptr, *field[8];
+        for (int j = 0; j < 8; ++j) field[j] = strtok_r(j ? nullptr : line, "\t\r\n", &save);
+        for (int j = 0; j < 8; ++j)
+            if (!field[j]) {
+                std::fclose(f);
+                throw DErr(UPDC_ERR_ARG,
+                           "manifest requires science, weight, psf, psf_width, psf_height, vertices, background, pedestal");
+            }
+        Row r;
+        r.sci = field[0];
+        r.wht = field[1];
+        r.psf = field[2];
+        r.pw = (int)std::strtol(field[3], nullptr, 10);
+        r.ph = (int)std::strtol(field[4], nullptr, 10);
+        r.vert = field[5];
+        // the reference parses these with strtof (f32); the FP64 tiers keep strtod
+        r.bg_f = std::strtof(field[6], nullptr);
+        r.ped_f = std::strtof(field[7], nullptr);
+        r.bg = std::strtod(field[6], nullptr);
+        r.ped = std::strtod(field[7], nullptr);
+        r.links = r.vert + ".links";
+        if (r.pw < 1 || r.ph < 1) {
+            std::fclose(f);
+            throw DErr(UPDC_ERR_ARG, "invalid PSF dimensions in manifest");
+        }
+        const float mo = r.bg_f + r.ped_f;
+        if (!std::isfinite(r.bg_f) || !std::isfinite(r.ped_f) || r.ped_f < 0.0f || !std::isfinite(mo)) {
+            std::fclose(f);
+            throw DErr(UPDC_ERR_ARG, "invalid background/pedestal in manifest");
+        }
+        rows.push_back(r);
+    }
+    if (std::fgets(line, sizeof line, f)) {
+        std::fclose(f);
+        throw DErr(UPDC_ERR_ARG, "manifest has more rows than expected");
+    }
+    std::fclose(f);
+    return rows;
+}
+
+// updc_core.c:298-315
+template <class T> std::vector<T> read_exact(const std::string &path, size_t count, const char *label) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) throw DErr(UPDC_ERR_IO, std::string(label) + ": " + path + ": cannot open input");
+    std::vector<T> v(count);
+    if (std::fread(v.data(), sizeof(T), count, f) != count) {
+        std::fclose(f);
+        throw DErr(UPDC_ERR_IO, std::string(label) + ": " + path + ": short input");
+    }
+    if (std::fgetc(f) != EOF) {
+        std::fclose(f);
+        throw DErr(UPDC_ERR_IO, std::string(label) + ": " + path + ": input has unexpected trailing bytes");
+    }
+    std::fclose(f);
+    return v;
+}
+
+bool read_file(const std::string &path, std::vector<uint8_t> &out) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    struct stat st;
+    if (fstat(fileno(f), &st) != 0) {
+        std::fclose(f);
+        return false;
+    }
+    out.resize((size_t)st.st_size);
+    const bool ok = std::fread(out.data(), 1, out.size(), f) == out.size();
+    std::fclose(f);
+    return ok;
+}
+
+// tmp -> fsync -> rename (updc_core.c:151-157, 574-601)
+void write_atomic(const std::string &path, const void *data, size_t bytes, const char *what) {
+    const std::string tmp = path + ".build." + std::to_string((long)getpid());
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) throw DErr(UPDC_ERR_IO, std::string("cannot create ") + what);
+    if (bytes && std::fwrite(data, 1, bytes, f) != bytes) {
+        std::fclose(f);
+        throw DErr(UPDC_ERR_IO, std::string(what) + " write failed");
+    }
+    if (std::fflush(f) || fsync(fileno(f)) || std::fclose(f)) throw DErr(UPDC_ERR_IO, std::string(what) + " close failed");
+    if (std::rename(tmp.c_str(), path.c_str())) throw DErr(UPDC_ERR_IO, std::string(what) + " rename failed");
+}
+
+void check(int rc) {
+    if (rc != UPDC_OK) throw DErr(rc, updc_last_error());
+}
+
+struct CtxGuard {
+    updc_ctx *c = nullptr;
+    ~CtxGuard() {
+        if (c) updc_ctx_destroy(c);
+    }
+};
+
+int env_int(const char *name, int dflt) {
+    const char *v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
+}
+
+void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_t, int iterations,
+             const char *prefix, const char *wht_out, int fftw, int expw, int prew, int plan_measure, int autotune,
+             int retune, const char *wisdom, int save_every) {
+    (void)fftw; (void)expw; (void)plan_measure; (void)autotune; (void)retune; (void)wisdom;
+    if (W < 1 || H < 1 || iw < 1 || ih < 1 || n < 1 || iterations < 1 || fftw < 1 || expw < 1 || prew < 1 ||
+        save_every < 0)
+        throw DErr(UPDC_ERR_ARG, "invalid dimensions/exposures/iterations/workers/save_every");
+    const double t_start = now();
+    const size_t nout = (size_t)W * H, nin = (size_t)iw * ih;
+    if (nout > 0xffffffffull) throw DErr(UPDC_ERR_ARG, "target grid exceeds 32-bit link index capacity");
+    const char *pc = std::getenv("UPDC_PERSIST_CACHE");
+    const bool persist = pc && std::strcmp(pc, "0") != 0;
+    const bool quiet = env_int("UPDC_HIP_QUIET", 0) != 0;
+    std::printf("Persistent C preprocessing cache: %s\n", persist ? "enabled" : "disabled");
+    std::vector<Row> rows = parse_manifest(manifest, n);
+
+    // precision tier of the drop-in: reference float32 by default
+    int prec = UPDC_PREC_F32;
+    if (const char *p = std::getenv("UPDC_HIP_PRECISION")) {
+        if (!std::strcmp(p, "f64")) prec = UPDC_PREC_F64;
+        else if (!std::strcmp(p, "mixed")) prec = UPDC_PREC_MIXED;
+    }
+    unsigned flags = env_int("UPDC_HIP_GRAPH", 1) ? UPDC_FLAG_GRAPH : 0u;
+    if (quiet) flags |= UPDC_FLAG_QUIET;
+    CtxGuard g;
+    check(updc_ctx_create(&g.c, W, H, 0, 0, prec, flags, env_int("UPDC_HIP_DEVICE", 0)));
+    check(updc_ctx_set_preprocess_workers(g.c, prew));
+
+    double t0 = now();
+    double t_links = 0, t_data = 0;
+    for (int k = 0; k < n; ++k) {
+        const Row &r = rows[k];
+        // F32 keeps the reference's strtof values exactly; the FP64 tiers use strtod
+        const double bg = prec == UPDC_PREC_F32 ? (double)r.bg_f : r.bg;
+        const double ped = prec == UPDC_PREC_F32 ? (double)r.ped_f : r.ped;
+        double td = now();
+        std::vector<float> sci = read_exact<float>(r.sci, nin, "SCI raw");
+        std::vector<float> wht = read_exact<float>(r.wht, nin, "WHT raw");
+        std::vector<float> psf = read_exact<float>(r.psf, (size_t)r.pw * r.ph, "PSF raw");
+        t_data += now() - td;
+        double tl = now();
+        std::vector<uint8_t> img;
+        bool hit = false;
+        if (persist && read_file(r.links, img)) {
+            const int rc = updc_add_exposure(g.c, sci.data(), wht.data(), iw, ih, img.data(), img.size(),
+                                             psf.data(), r.pw, r.ph, psf_t, bg, ped);
+            if (rc == UPDC_OK) hit = true;
+            else if (rc != UPDC_ERR_FORMAT) check(rc);
+        }
+        if (!hit) {
+            std::vector<double> vert =
+                read_exact<double>(r.vert, 2 * (size_t)(iw + 1) * (ih + 1), "vertex map");
+            check(updc_add_exposure_vertices(g.c, sci.data(), wht.data(), iw, ih, vert.data(), psf.data(), r.pw,
+                                             r.ph, psf_t, bg, ped));
+            size_t bytes = 0;
+            check(updc_linkmap_size(g.c, k, &bytes));
+            img.resize(bytes);
+            check(updc_get_linkmap(g.c, k, img.data(), bytes));
+            write_atomic(r.links, img.data(), bytes, "LinkMap");
+        }
+        uint32_t enc = 0;
+        std::memcpy(&enc, img.data() + 8, 4);
+        uint64_t nl = 0;
+        std::memcpy(&nl, img.data() + 24, 8);
+        std::printf("AreaLink exposure %d: %s%llu overlaps; encoding=%s\n", k + 1, hit ? "persistent cache hit, " : "",
+                    (unsigned long long)nl, enc == UPDC_LINKMAP_DELTA16 ? "DELTA16" : "ABS32");
+        t_links += now() - tl;
+    }
+    const double t_setup_in = now() - t0;
+    t0 = now();
+    check(updc_prepare(g.c));
+    const double t_prepare = now() - t0;
+    double ws = 0, ms = 0;
+    int fw = 0, fh = 0;
+    uint64_t masked = 0;
+    check(updc_get_scalars(g.c, &ws, &ms, &fw, &fh, &masked));
+    std::printf("Background-aware UPDC (MI355X): %d exposures, input %dx%d, target %dx%d, FFT %dx%d, iterations %d\n",
+                n, iw, ih, W, H, fw, fh, iterations);
+    std::printf("WHT median normalization %.9g; %llu masked pixels; max H^T W sensitivity %.9g\n", ws,
+                (unsigned long long)masked, ms);
+    std::fflush(stdout);
+
+    const std::string freeze_path = std::string(prefix) + "_mufreeze.csv";
+    FILE *fcsv = std::fopen(freeze_path.c_str(), "w");
+    if (!fcsv) throw DErr(UPDC_ERR_IO, "cannot write freeze CSV");
+    std::fprintf(fcsv, "iteration,exposure,n_new_frozen,n_total_frozen,min_raw_mu,mu_marker,frozen_ratio\n");
+    std::vector<float> est(nout);
+    std::vector<double> iter_s(iterations, 0.0), out_s(iterations, 0.0);
+    int done = 0;
+    try {
+        while (done < iterations) {
+            // run up to the next saved iteration in one call
+            int next = iterations;
+            if (save_every > 0) next = std::min(iterations, (done / save_every + 1) * save_every);
+            const int chunk = next - done;
+            std::vector<updc_freeze_row> fr((size_t)chunk * n);
+            std::vector<updc_iter_info> info(chunk);
+            const double tc = now();
+            const int rc = updc_iterate(g.c, chunk, fr.data(), info.data());
+            const double el = now() - tc;
+            for (int i = 0; i < chunk; ++i) {
+                for (int k = 0; k < n; ++k) {
+                    const updc_freeze_row &r = fr[(size_t)i * n + k];
+                    if (rc != UPDC_OK && r.n_new_frozen == 0 && r.n_total_frozen == 0 && std::isinf(r.min_raw_mu))
+                        continue;
+                    std::fprintf(fcsv, "%d,%d,%llu,%llu,%.9g,%.9g,1\n", done + i + 1, k + 1,
+                                 (unsigned long long)r.n_new_frozen, (unsigned long long)r.n_total_frozen,
+                                 (double)(float)r.min_raw_mu, (double)1.0e-12f);
+                }
+                iter_s[done + i] = el / chunk;
+                if (!quiet)
+                    std::printf("UPDC iteration %d/%d: signed sum %.9g, negative %llu/%zu, frozen +%llu total %llu, "
+                                "min raw mu %.9g, GPU %.3f ms\n",
+                                done + i + 1, iterations, info[i].signed_sum, (unsigned long long)info[i].n_negative,
+                                nout, (unsigned long long)info[i].n_new_frozen,
+                                (unsigned long long)info[i].n_total_frozen, info[i].min_raw_mu, info[i].ms_total);
+            }
+            check(rc);
+            done = next;
+            const double to = now();
+            check(updc_get_estimate(g.c, est.data()));
+            char path[4096];
+            if (std::snprintf(path, sizeof path, "%s_%02d.bin", prefix, done) >= (int)sizeof path)
+                throw DErr(UPDC_ERR_ARG, "iteration output path is too long");
+            FILE *f = std::fopen(path, "wb");
+            if (!f) throw DErr(UPDC_ERR_IO, "cannot create iteration output");
+            if (std::fwrite(est.data(), 4, nout, f) != nout) {
+                std::fclose(f);
+                throw DErr(UPDC_ERR_IO, "iteration output write failed");
+            }
+            if (std::fclose(f)) throw DErr(UPDC_ERR_IO, "iteration output close failed");
+            std::printf("UPDC iteration %d raw output: %s\n", done, path);
+            out_s[done - 1] = now() - to;
+        }
+    } catch (...) {
+        std::fclose(fcsv);
+        throw;
+    }
+    if (std::fclose(fcsv)) throw DErr(UPDC_ERR_IO, "freeze CSV close failed");
+
+    std::vector<float> wht(nout);
+    check(updc_get_wht(g.c, wht.data()));
+    {
+        FILE *f = std::fopen(wht_out, "wb");
+        if (!f) throw DErr(UPDC_ERR_IO, "cannot create WHT output");
+        if (std::fwrite(wht.data(), 4, nout, f) != nout) {
+            std::fclose(f);
+            throw DErr(UPDC_ERR_IO, "WHT output write failed");
+        }
+        if (std::fclose(f)) throw DErr(UPDC_ERR_IO, "WHT output close failed");
+    }
+    // timing CSV (updc_core.c:1487-1508), GPU stage names
+    const std::string tpath = std::string(prefix) + "_timing.csv";
+    FILE *tf = std::fopen(tpath.c_str(), "w");
+    if (!tf) throw DErr(UPDC_ERR_IO, "cannot write timing CSV");
+    std::fprintf(tf, "stage,wall_seconds\ninputs_and_linkmaps,%.6f\npolygon_links,%.6f\ndata_cache,%.6f\nprepare,%.6f\n",
+                 t_setup_in, t_links, t_data, t_prepare);
+    char names[16][32];
+    double sms[16];
+    int sn[16], cnt = 16;
+    if (updc_get_stage_times(g.c, names, sms, sn, &cnt) == UPDC_OK)
+        for (int i = 0; i < cnt; ++i) std::fprintf(tf, "last_call_avg_%s_ms,%.6f\n", names[i], sms[i]);
+    for (int i = 0; i < iterations; ++i) {
+        std::fprintf(tf, "iteration_%02d_output,%.6f\n", i + 1, out_s[i]);
+        std::fprintf(tf, "iteration_%02d_total,%.6f\n", i + 1, iter_s[i] + out_s[i]);
+    }
+    std::fprintf(tf, "total_solver,%.6f\n", now() - t_start);
+    if (std::fclose(tf)) throw DErr(UPDC_ERR_IO, "timing CSV close failed");
+    std::printf("FFT transforms per iteration: %d (2N+2)\n", 2 * n + 2);
+    std::fflush(stdout);
+}
+
+}  // namespace
+
+namespace updc { extern thread_local std::string g_last_error; }
+
+extern "C" int updc_all(const char *manifest, int width, int height, int input_width, int input_height,
+                        int exposures, int transpose_psf, int iterations, const char *iter_prefix,
+                        const char *weight_output, int fft_worker_count, int exposure_worker_count,
+                        int preprocess_worker_count, int plan_measure, int autotune, int retune_fft,
+                        const char *wisdom_path, int save_every) {
+    int code = UPDC_OK;
+    try {
+        if (!manifest || !iter_prefix || !weight_output) throw DErr(UPDC_ERR_ARG, "null path");
+        run_all(manifest, width, height, input_width, input_height, exposures, transpose_psf, iterations,
+                iter_prefix, weight_output, fft_worker_count, exposure_worker_count, preprocess_worker_count,
+                plan_measure, autotune, retune_fft, wisdom_path ? wisdom_path : "", save_every);
+        updc::g_last_error.clear();
+    } catch (const DErr &e) {
+        updc::g_last_error = e.what();
+        code = e.code;
+    } catch (const std::bad_alloc &) {
+        updc::g_last_error = "out of memory";
+        code = UPDC_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        updc::g_last_error = e.what();
+        code = UPDC_ERR_ARG;
+    }
+    std::fflush(stdout);
+    return code;
+}

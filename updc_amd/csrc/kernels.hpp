@@ -1,0 +1,58 @@
+// Device data structures shared by the kernels and the host orchestration.
+#pragma once
+#include "common.hpp"
+
+// LinkMap v2 encodings (updc/c_core/updc_core.c:44-47).
+#define UPDC_LM_MAGIC 0x324d4c55u
+#define UPDC_LM_VERSION 2u
+#define UPDC_LM_ABS32 1u
+#define UPDC_LM_DELTA16 2u
+
+// Per-exposure view handed to every batched kernel.  Detector-side arrays
+// are the ULM2 CSR arrays exactly as stored (no re-encoding), plus the
+// target-major (CSC) transpose built once in updc_prepare.
+struct ExpDev {
+    uint32_t nin;       // detector pixels
+    uint32_t encoding;  // UPDC_LM_DELTA16 / UPDC_LM_ABS32
+    const uint32_t *off;   // [nin+1]
+    const uint32_t *base;  // [nin]      (DELTA16)
+    const uint16_t *d16;   // [nlinks]   (DELTA16)
+    const uint32_t *t32;   // [nlinks]   (ABS32)
+    const float *frac;     // [nlinks]
+    const uint32_t *toff;  // [nout+1]   CSC offsets
+    const uint2 *tent;     // [nlinks']  CSC entries {pixel, frac bits}, ascending pixel per target
+    size_t pix;            // start of this exposure in the flattened per-pixel arrays
+    double mean_offset, pedestal, background;     // manifest values (f64 parse)
+    float mean_offset_f, pedestal_f, background_f; // the reference's f32 values (updc_core.c:447-449)
+};
+
+__device__ __forceinline__ uint32_t lm_target(const ExpDev &e, uint32_t p, uint32_t z) {
+    return e.encoding == UPDC_LM_DELTA16 ? e.base[p] + (uint32_t)e.d16[z] : e.t32[z];
+}
+
+// Per-(iteration, exposure) FreezeMu diagnostics written by the gather kernel
+// (the mufreeze CSV row, updc_core.c:1425-1431).
+struct ExpDiag {
+    unsigned long long n_new;
+    unsigned long long n_total;
+    unsigned long long min_key;  // dkey(min raw_mu)
+};
+// Per-iteration diagnostics of the update kernel (updc_core.c:1438-1456) and
+// error word.  err bits: 1 non-finite forward mean, 2 non-finite ratio,
+// 4 non-finite correction factor, 8 substantially negative correction,
+// 16 non-finite estimate.  err_index keeps the smallest failing index.
+struct IterDiag {
+    double signed_sum;
+    unsigned long long n_negative;
+    unsigned int err;
+    unsigned int pad;
+    unsigned long long err_index;
+};
+#define UPDC_E_MU 1u
+#define UPDC_E_RATIO 2u
+#define UPDC_E_FACTOR 4u
+#define UPDC_E_NEGCORR 8u
+#define UPDC_E_ESTIMATE 16u
+#define UPDC_E_DRIZZLE 32u
+#define UPDC_E_WEIGHT 64u
+#define UPDC_E_YEFF 128u

@@ -1,0 +1,1208 @@
+// Context, setup and iteration driver of the MI355X UPDC library (C ABI).
+//
+// One updc_ctx holds every exposure's device data for the whole solve, so the
+// iteration never touches the host: LinkMaps and their CSC transposes, PSF
+// spectra, normalised SCI/WHT, FreezeMu masks, the estimate and the
+// denominator stay resident in HBM (288 GB per MI355X holds every workload in
+// the survey with room to spare).  One iteration is eight batched launches
+// (every exposure inside each launch) plus one bookkeeping kernel, optionally
+// captured once as a hipGraph and replayed.
+#include "updc_hip.h"
+#include "fft.hpp"
+#include "stages.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace updc {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) throw Error(UPDC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+thread_local std::string g_last_error;
+
+template <class T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T *alloc(size_t count) {
+        reset();
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            throw Error(UPDC_ERR_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+        return p;
+    }
+};
+
+// ------------------------------------------------------------------ host geometry
+// Host copy of the polygon policy (overlap.c:72-116) for the C ABI symbols and
+// the self-test; the GPU builder has its own device copy in stages.hip.
+namespace geo {
+struct P { double x, y; };
+static bool in(P p, int s, double v) { return s == 0 ? p.x >= v : s == 1 ? p.x <= v : s == 2 ? p.y >= v : p.y <= v; }
+static P cross(P a, P b, int s, double v) {
+    P r = a;
+    if (s < 2) {
+        const double d = b.x - a.x;
+        if (std::fabs(d) > 1.0e-15) { const double t = (v - a.x) / d; r.x = v; r.y = a.y + t * (b.y - a.y); }
+    } else {
+        const double d = b.y - a.y;
+        if (std::fabs(d) > 1.0e-15) { const double t = (v - a.y) / d; r.x = a.x + t * (b.x - a.x); r.y = v; }
+    }
+    return r;
+}
+static float area(const P *p, int n) {
+    if (n < 3) return 0.0f;
+    const double ox = p[0].x, oy = p[0].y;
+    float acc = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        const int j = (i + 1) % n;
+        const float xi = (float)(p[i].x - ox), yi = (float)(p[i].y - oy);
+        const float xj = (float)(p[j].x - ox), yj = (float)(p[j].y - oy);
+        acc += std::fmaf(xi, yj, -(xj * yi));
+    }
+    return 0.5f * std::fabs(acc);
+}
+static float overlap(const P q[4], int x, int y) {
+    P a[16], b[16];
+    P *inp = a, *out = b;
+    int n = 4;
+    for (int j = 0; j < 4; ++j) inp[j] = q[j];
+    const double lim[4] = {(double)x - 0.5, (double)x + 0.5, (double)y - 0.5, (double)y + 0.5};
+    for (int s = 0; s < 4; ++s) {
+        int m = 0;
+        P prev = inp[n - 1];
+        bool pin = in(prev, s, lim[s]);
+        for (int i = 0; i < n; ++i) {
+            const P cur = inp[i];
+            const bool cin = in(cur, s, lim[s]);
+            if (cin) {
+                if (!pin && m < 16) out[m++] = cross(prev, cur, s, lim[s]);
+                if (m < 16) out[m++] = cur;
+            } else if (pin && m < 16) {
+                out[m++] = cross(prev, cur, s, lim[s]);
+            }
+            prev = cur;
+            pin = cin;
+        }
+        n = m;
+        if (n < 3) return 0.0f;
+        std::swap(inp, out);
+    }
+    return area(inp, n);
+}
+}  // namespace geo
+
+// ------------------------------------------------------------------ helpers
+static bool seven_smooth(long n) {
+    if (n < 1) return false;
+    for (int p : {2, 3, 5, 7}) while (n % p == 0) n /= p;
+    return n == 1;
+}
+static int next_even_smooth(int n) {
+    if (n < 2) n = 2;
+    for (;; ++n)
+        if (!(n & 1) && seven_smooth(n)) return n;
+}
+static int next_smooth(int n) {
+    if (n < 1) n = 1;
+    for (;; ++n)
+        if (seven_smooth(n)) return n;
+}
+// Neumaier-compensated sums (updc_core.c:777-789): f32 as the reference, f64
+// for the FP64 tier.  Return NaN on an invalid pixel.
+template <class T> static T psf_sum(const float *psf, size_t n) {
+    T s = 0, c = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const T v = (T)psf[i];
+        if (!std::isfinite(psf[i]) || psf[i] < 0.0f) return (T)NAN;
+        const T t = s + v;
+        if (std::fabs(s) >= std::fabs(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+    }
+    return s + c;
+}
+static size_t align4(size_t v) { return (v + 3u) & ~(size_t)3u; }
+static size_t linkmap_bytes(uint64_t nin, uint64_t nlinks, uint32_t enc) {
+    size_t pos = 64 + (size_t)(nin + 1) * 4u;
+    if (enc == UPDC_LM_DELTA16) return align4(pos + nin * 4u + nlinks * 2u) + nlinks * 4u;
+    if (enc == UPDC_LM_ABS32) return pos + nlinks * 8u;
+    return 0;
+}
+
+struct HostExp {
+    int iw = 0, ih = 0;
+    size_t nin = 0, pix = 0;
+    std::vector<float> sci, wht, psf;
+    int pw = 0, ph = 0, psf_t = 0;
+    double background = 0, pedestal = 0;
+    float background_f = 0, pedestal_f = 0, mean_offset_f = 0;
+    DevBuf<uint8_t> lm;
+    size_t lm_bytes = 0;
+    uint32_t encoding = 0;
+    uint64_t nlinks = 0;
+    DevBuf<uint32_t> toff;
+    DevBuf<uint2> tent;
+    uint64_t ncsc = 0;
+    ExpDev view() const {
+        ExpDev e{};
+        e.nin = (uint32_t)nin;
+        e.encoding = encoding;
+        const uint8_t *b = lm.p;
+        size_t pos = 64;
+        e.off = (const uint32_t *)(b + pos);
+        pos += (nin + 1) * 4u;
+        if (encoding == UPDC_LM_DELTA16) {
+            e.base = (const uint32_t *)(b + pos);
+            pos += nin * 4u;
+            e.d16 = (const uint16_t *)(b + pos);
+            pos = align4(pos + nlinks * 2u);
+        } else {
+            e.t32 = (const uint32_t *)(b + pos);
+            pos += nlinks * 4u;
+        }
+        e.frac = (const float *)(b + pos);
+        e.toff = toff.p;
+        e.tent = tent.p;
+        e.pix = pix;
+        e.background = background;
+        e.pedestal = pedestal;
+        e.mean_offset = background + pedestal;
+        e.background_f = background_f;
+        e.pedestal_f = pedestal_f;
+        e.mean_offset_f = mean_offset_f;
+        return e;
+    }
+};
+
+struct Ctx;
+struct PipeBase {
+    virtual ~PipeBase() = default;
+    virtual void prepare(Ctx &c) = 0;
+    virtual void iterate(Ctx &c, int n, updc_freeze_row *rows, updc_iter_info *info) = 0;
+    virtual void get_estimate(Ctx &c, double *d, float *f) = 0;
+    virtual void set_estimate(Ctx &c, const double *d) = 0;
+    virtual void get_wht(Ctx &c, float *out) = 0;
+    virtual void get_denominator(Ctx &c, double *out) = 0;
+    virtual void get_initial(Ctx &c, double *est, double *cov) = 0;
+    virtual void get_psf_spectrum(Ctx &c, int k, double *out) = 0;
+    virtual void stage_forward(Ctx &c, const double *x, double *blur) = 0;
+    virtual void stage_model(Ctx &c, const double *x, double *pred) = 0;
+    virtual void stage_detector(Ctx &c, int k, const double *blur, double *wr, double *scatter, double *stats) = 0;
+    virtual void stage_adjoint(Ctx &c, const double *v, double *out) = 0;
+    virtual void stage_scatter(Ctx &c, int k, const double *v, double *out) = 0;
+    virtual void fft_r2c(Ctx &c, const double *img, double *spec) = 0;
+    virtual void fft_c2r(Ctx &c, const double *spec, double *img) = 0;
+    virtual void get_frozen(Ctx &c, int k, uint8_t *out) = 0;
+    virtual void reset_frozen(Ctx &c) = 0;
+    double max_sens = 0;
+    std::vector<std::string> stage_names;
+    std::vector<double> stage_ms;
+    std::vector<int> stage_n;
+};
+
+struct Ctx {
+    int W = 0, H = 0, FW = 0, FH = 0, prec = 0, device = 0;
+    unsigned flags = 0;
+    int workers = 1;
+    std::vector<std::unique_ptr<HostExp>> ex;
+    hipStream_t st = nullptr;
+    bool prepared = false;
+    float weight_scale = 0;
+    unsigned long long masked = 0;
+    std::unique_ptr<PipeBase> pipe;
+    size_t nout() const { return (size_t)W * H; }
+    size_t npix() const {
+        size_t s = 0;
+        for (auto &e : ex) s += e->nin;
+        return s;
+    }
+    bool quiet() const { return flags & UPDC_FLAG_QUIET; }
+};
+
+// ------------------------------------------------------------------ pipeline
+template <class TF, class TA> struct Pipe : PipeBase {
+    Fft2<TF> fft;
+    DevBuf<ExpDev> d_ex;
+    std::vector<ExpDev> h_ex;
+    DevBuf<TF> sci, w, wr, blur, x, C, cov;
+    DevBuf<TA> D;
+    DevBuf<uint8_t> frozen;
+    DevBuf<Cx<TF>> psf_spec, spec_x, spec_k;
+    DevBuf<ExpDiag> diag_cur, diag_hist;
+    DevBuf<IterDiag> idiag_cur, idiag_hist;
+    DevBuf<double> partials;
+    DevBuf<unsigned long long> negs, key;
+    DevBuf<unsigned int> ticket, ctr, err;
+    int hist_cap = 0;
+    uint32_t max_nin = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_cap = -1;
+
+    ~Pipe() override {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        fft.release();
+    }
+
+    int N(Ctx &c) const { return (int)c.ex.size(); }
+    TF scale(Ctx &c) const { return (TF)1 / (TF)((size_t)c.FW * c.FH); }
+
+    // ---- PSF spectra P^_k for every exposure into `out` (N x FH x NCP).
+    template <class TD> static void psf_spectra(Ctx &c, Fft2<TD> &f, Cx<TD> *out) {
+        const FftShape &s = f.s;
+        DevBuf<TD> img;
+        img.alloc((size_t)s.FW * s.FH);
+        DevBuf<float> raw;
+        for (size_t k = 0; k < c.ex.size(); ++k) {
+            HostExp &e = *c.ex[k];
+            const size_t cnt = (size_t)e.pw * e.ph;
+            TD inv;
+            if (sizeof(TD) == 4) {
+                const float sum = psf_sum<float>(e.psf.data(), cnt);
+                if (!(sum > 0.0f) || !std::isfinite(sum)) throw Error(UPDC_ERR_NUMERIC, "PSF sum is zero/non-finite");
+                inv = (TD)(1.0f / sum);
+            } else {
+                const double sum = psf_sum<double>(e.psf.data(), cnt);
+                if (!(sum > 0.0) || !std::isfinite(sum)) throw Error(UPDC_ERR_NUMERIC, "PSF sum is zero/non-finite");
+                inv = (TD)(1.0 / sum);
+            }
+            raw.alloc(cnt);
+            HIPCHK(hipMemcpyAsync(raw.p, e.psf.data(), cnt * sizeof(float), hipMemcpyHostToDevice, c.st));
+            HIPCHK(fill<TD>(img.p, (TD)0, img.n, c.st));
+            HIPCHK(psf_wrap<TD>(raw.p, e.pw, e.ph, e.psf_t, s.FW, s.FH, inv, img.p, c.st));
+            Cx<TD> *o = out + k * (size_t)s.FH * s.NCP;
+            HIPCHK(f.row_r2c(ROW_PLAIN, img.p, 0, s.FW, s.FW, nullptr, nullptr, 0, o, 0, s.FH, 1, c.st));
+            HIPCHK(f.template colpass<TD>(COL_FWD, o, 0, o, 0, nullptr, 0, s.FH, s.FH, 1, c.st));
+            HIPCHK(hipStreamSynchronize(c.st));
+        }
+    }
+
+    // ---- D = crop(C2R(sum_k conj(P^_k) R2C(pad(S_k^T w_k)))) / nf (updc_core.c:1026-1086).
+    template <class TD>
+    static void denominator(Ctx &c, Fft2<TD> &f, const Cx<TD> *pspec, const ExpDev *dex, const TD *wts, TD *Dout) {
+        const FftShape &s = f.s;
+        const int n = (int)c.ex.size();
+        DevBuf<Cx<TD>> sk, sx;
+        sk.alloc((size_t)n * c.H * s.NCP);
+        sx.alloc((size_t)c.H * s.NCP);
+        HIPCHK(f.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, dex, wts, c.W, sk.p, (size_t)c.H * s.NCP, c.H, n, c.st));
+        HIPCHK(f.template colpass<TD>(COL_ADJ, sk.p, (size_t)c.H * s.NCP, sx.p, 0, pspec, (size_t)s.FH * s.NCP, c.H,
+                                      c.H, n, c.st));
+        const TD sc = (TD)1 / (TD)((size_t)s.FW * s.FH);
+        HIPCHK(f.row_c2r(sx.p, 0, Dout, 0, c.W, c.W, sc, c.H, 1, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+    }
+
+    void build_csc(Ctx &c) {
+        const size_t nout = c.nout();
+        DevBuf<uint32_t> tcnt, cursor;
+        tcnt.alloc(nout);
+        cursor.alloc(nout + 1);
+        DevBuf<uint8_t> scratch;
+        scratch.alloc(scan_scratch_bytes(nout));
+        DevBuf<unsigned long long> tot;
+        tot.alloc(1);
+        DevBuf<ExpDev> one;
+        one.alloc(1);
+        for (size_t k = 0; k < c.ex.size(); ++k) {
+            HostExp &e = *c.ex[k];
+            e.toff.alloc(nout + 1);
+            ExpDev v = e.view();
+            HIPCHK(hipMemcpyAsync(one.p, &v, sizeof(ExpDev), hipMemcpyHostToDevice, c.st));
+            HIPCHK(fill<uint32_t>(tcnt.p, 0u, nout, c.st));
+            HIPCHK(csc_count<TF>(one.p, (uint32_t)e.nin, w.p + e.pix, tcnt.p, c.st));
+            HIPCHK(scan_u32(tcnt.p, e.toff.p, nout, scratch.p, tot.p, c.st));
+            unsigned long long total = 0;
+            HIPCHK(hipMemcpyAsync(&total, tot.p, sizeof(total), hipMemcpyDeviceToHost, c.st));
+            HIPCHK(hipStreamSynchronize(c.st));
+            if (total > 0xffffffffull) throw Error(UPDC_ERR_ARG, "CSC transpose exceeds uint32 capacity");
+            e.ncsc = total;
+            e.tent.alloc(total);
+            HIPCHK(hipMemcpyAsync(cursor.p, e.toff.p, nout * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.st));
+            v = e.view();
+            HIPCHK(hipMemcpyAsync(one.p, &v, sizeof(ExpDev), hipMemcpyHostToDevice, c.st));
+            HIPCHK(csc_fill<TF>(one.p, (uint32_t)e.nin, w.p + e.pix, cursor.p, e.tent.p, c.st));
+            HIPCHK(csc_sort(e.toff.p, e.tent.p, nout, c.st));
+            HIPCHK(hipStreamSynchronize(c.st));
+        }
+    }
+
+    void upload_ex(Ctx &c) {
+        h_ex.clear();
+        for (auto &e : c.ex) h_ex.push_back(e->view());
+        d_ex.alloc(h_ex.size());
+        HIPCHK(hipMemcpyAsync(d_ex.p, h_ex.data(), h_ex.size() * sizeof(ExpDev), hipMemcpyHostToDevice, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+    }
+
+    void prepare(Ctx &c) override {
+        const int n = N(c);
+        const size_t nout = c.nout(), npix = c.npix();
+        std::string e = fft.init(c.FW, c.FH);
+        if (!e.empty()) throw Error(UPDC_ERR_ARG, e);
+        const FftShape &s = fft.s;
+        max_nin = 0;
+        for (auto &x : c.ex) max_nin = std::max<uint32_t>(max_nin, (uint32_t)x->nin);
+
+        // weights (updc_core.c:378-424): median of the p % 64 == 0 sample.
+        std::vector<float> sample;
+        for (auto &x : c.ex)
+            for (size_t p = 0; p < x->nin; p += 64)
+                if (std::isfinite(x->wht[p]) && x->wht[p] > 0) sample.push_back(x->wht[p]);
+        if (sample.empty()) throw Error(UPDC_ERR_NUMERIC, "all input WHT pixels are zero");
+        std::sort(sample.begin(), sample.end());
+        c.weight_scale = sample[sample.size() / 2];
+        if (!(c.weight_scale > 0) || !std::isfinite(c.weight_scale))
+            throw Error(UPDC_ERR_NUMERIC, "invalid WHT normalization");
+        sci.alloc(npix);
+        w.alloc(npix);
+        wr.alloc(npix);
+        frozen.alloc(npix);
+        err.alloc(1);
+        key.alloc(1);
+        DevBuf<unsigned long long> masked;
+        masked.alloc(1);
+        HIPCHK(fill<unsigned int>(err.p, 0u, 1, c.st));
+        HIPCHK(fill<unsigned long long>(masked.p, 0ull, 1, c.st));
+        {
+            DevBuf<float> rs, rw;
+            for (auto &x : c.ex) {
+                rs.alloc(x->nin);
+                rw.alloc(x->nin);
+                HIPCHK(hipMemcpyAsync(rs.p, x->sci.data(), x->nin * 4, hipMemcpyHostToDevice, c.st));
+                HIPCHK(hipMemcpyAsync(rw.p, x->wht.data(), x->nin * 4, hipMemcpyHostToDevice, c.st));
+                if (sizeof(TF) == 4)
+                    HIPCHK(normalize_weights<TF>(rs.p, rw.p, x->nin, c.weight_scale, (double)c.weight_scale,
+                                                 x->pedestal_f, x->pedestal, sci.p + x->pix, w.p + x->pix, masked.p,
+                                                 err.p, c.st));
+                else
+                    HIPCHK(normalize_weights<TF>(rs.p, rw.p, x->nin, c.weight_scale, (double)c.weight_scale,
+                                                 x->pedestal_f, x->pedestal, sci.p + x->pix, w.p + x->pix, masked.p,
+                                                 err.p, c.st));
+                HIPCHK(hipStreamSynchronize(c.st));
+            }
+        }
+        unsigned int ebits = 0;
+        HIPCHK(hipMemcpy(&ebits, err.p, 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&c.masked, masked.p, 8, hipMemcpyDeviceToHost));
+        if (ebits & UPDC_E_WEIGHT) throw Error(UPDC_ERR_NUMERIC, "invalid normalized WHT");
+        if (ebits & UPDC_E_YEFF) throw Error(UPDC_ERR_NUMERIC, "negative/non-finite y_eff reached solver");
+
+        build_csc(c);
+        upload_ex(c);
+
+        // PSF spectra
+        psf_spec.alloc((size_t)n * s.FH * s.NCP);
+        psf_spectra<TF>(c, fft, psf_spec.p);
+
+        // initial drizzle (updc_core.c:905-983)
+        x.alloc(nout);
+        cov.alloc(nout);
+        HIPCHK(fill<unsigned int>(err.p, 0u, 1, c.st));
+        HIPCHK(drizzle<TF>(d_ex.p, n, sci.p, w.p, std::max(1, std::min(c.workers, n)), nout, x.p, cov.p, err.p, c.st));
+        HIPCHK(hipMemcpyAsync(&ebits, err.p, 4, hipMemcpyDeviceToHost, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        if (ebits & UPDC_E_DRIZZLE) throw Error(UPDC_ERR_NUMERIC, "non-finite signed initial estimate");
+
+        // denominator
+        D.alloc(nout);
+        if (sizeof(TA) == sizeof(TF)) {
+            denominator<TF>(c, fft, psf_spec.p, d_ex.p, w.p, (TF *)D.p);
+        } else {
+            // mixed tier: the denominator is computed entirely in FP64
+            Fft2<double> f64;
+            std::string e2 = f64.init(c.FW, c.FH);
+            if (!e2.empty()) throw Error(UPDC_ERR_ARG, e2);
+            try {
+                DevBuf<Cx<double>> ps;
+                ps.alloc((size_t)n * f64.s.FH * f64.s.NCP);
+                psf_spectra<double>(c, f64, ps.p);
+                DevBuf<double> wd;
+                wd.alloc(npix);
+                HIPCHK((convert<TF, double>(w.p, wd.p, npix, c.st)));
+                denominator<double>(c, f64, ps.p, d_ex.p, wd.p, (double *)D.p);
+            } catch (...) {
+                f64.release();
+                throw;
+            }
+            f64.release();
+        }
+        HIPCHK(fill<unsigned long long>(key.p, 0ull, 1, c.st));
+        HIPCHK(max_key<TA>(D.p, nout, key.p, c.st));
+        unsigned long long kk = 0;
+        HIPCHK(hipMemcpyAsync(&kk, key.p, 8, hipMemcpyDeviceToHost, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        max_sens = std::max(0.0, dkey_inv(kk));
+        if (!(max_sens > 0) || !std::isfinite(max_sens)) throw Error(UPDC_ERR_NUMERIC, "invalid H^T W sensitivity");
+
+        // iteration buffers
+        blur.alloc((size_t)n * nout);
+        C.alloc(nout);
+        spec_x.alloc((size_t)std::max(c.H, s.FH) * s.NCP);
+        spec_k.alloc((size_t)n * c.H * s.NCP);
+        HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, npix, c.st));
+        diag_cur.alloc(n);
+        idiag_cur.alloc(1);
+        partials.alloc(update_blocks());
+        negs.alloc(update_blocks());
+        ticket.alloc(1);
+        ctr.alloc(1);
+        HIPCHK(fill<unsigned int>(ticket.p, 0u, 1, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        stage_names = {"row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze",
+                       "row_r2c_gather", "col_adj_accum", "row_c2r_corr", "update"};
+    }
+
+    // One iteration's launches (identical eager or under graph capture).
+    void launch_iteration(Ctx &c, hipEvent_t *ev) {
+        const int n = N(c);
+        const FftShape &s = fft.s;
+        const size_t nout = c.nout();
+        const size_t kstride = (size_t)c.H * s.NCP;
+        const TF sc = scale(c);
+        auto mark = [&](int i) { if (ev) HIPCHK(hipEventRecord(ev[i], c.st)); };
+        mark(0);
+        HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
+        mark(1);
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, (size_t)s.FH * s.NCP,
+                                        c.H, c.H, n, c.st));
+        mark(2);
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, sc, c.H, n, c.st));
+        mark(3);
+        HIPCHK((gather_freeze<TF, TA>(d_ex.p, 0, n, max_nin, blur.p, nout, sci.p, w.p, frozen.p, wr.p, diag_cur.p,
+                                      idiag_cur.p, c.st)));
+        mark(4);
+        HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, n, c.st));
+        mark(5);
+        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, (size_t)s.FH * s.NCP,
+                                        c.H, c.H, n, c.st));
+        mark(6);
+        HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
+        mark(7);
+        HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
+        mark(8);
+        HIPCHK(commit_diag(diag_cur.p, n, idiag_cur.p, diag_hist.p, idiag_hist.p, hist_cap, ctr.p, c.st));
+    }
+
+    void iterate(Ctx &c, int niter, updc_freeze_row *rows, updc_iter_info *info) override {
+        const int n = N(c);
+        if (niter > hist_cap) {
+            diag_hist.alloc((size_t)niter * n);
+            idiag_hist.alloc(niter);
+            hist_cap = niter;
+            if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+        }
+        HIPCHK(init_diag(diag_cur.p, n, idiag_cur.p, 1, c.st));
+        HIPCHK(init_diag(diag_hist.p, niter * n, idiag_hist.p, niter, c.st));
+        HIPCHK(fill<unsigned int>(ctr.p, 0u, 1, c.st));
+        const bool use_graph = (c.flags & UPDC_FLAG_GRAPH) != 0;
+        std::vector<hipEvent_t> ev;
+        const int nev = 9;
+        hipEvent_t it0, it1;
+        HIPCHK(hipEventCreate(&it0));
+        HIPCHK(hipEventCreate(&it1));
+        std::vector<double> acc(8, 0.0);
+        if (!use_graph) {
+            ev.resize((size_t)nev * niter);
+            for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        }
+        try {
+            if (use_graph && !graph) {
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_iteration(c, nullptr);
+                } catch (...) {
+                    hipGraph_t dummy;
+                    (void)hipStreamEndCapture(c.st, &dummy);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(c.st, &g));
+                HIPCHK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+                HIPCHK(hipGraphDestroy(g));
+            }
+            HIPCHK(hipEventRecord(it0, c.st));
+            for (int it = 0; it < niter; ++it) {
+                if (use_graph) HIPCHK(hipGraphLaunch(graph, c.st));
+                else launch_iteration(c, ev.data() + (size_t)it * nev);
+            }
+            HIPCHK(hipEventRecord(it1, c.st));
+            HIPCHK(hipStreamSynchronize(c.st));
+        } catch (...) {
+            for (auto &e : ev) (void)hipEventDestroy(e);
+            (void)hipEventDestroy(it0);
+            (void)hipEventDestroy(it1);
+            throw;
+        }
+        float total_ms = 0;
+        HIPCHK(hipEventElapsedTime(&total_ms, it0, it1));
+        stage_ms.assign(9, 0.0);
+        stage_n.assign(9, 0);
+        if (!use_graph) {
+            for (int it = 0; it < niter; ++it)
+                for (int s = 0; s < 8; ++s) {
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, ev[(size_t)it * nev + s], ev[(size_t)it * nev + s + 1]));
+                    stage_ms[s] += ms;
+                    stage_n[s] += 1;
+                }
+            for (int s = 0; s < 8; ++s) stage_ms[s] /= std::max(1, stage_n[s]);
+        }
+        stage_ms[8] = total_ms / std::max(1, niter);
+        stage_n[8] = niter;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(it0);
+        (void)hipEventDestroy(it1);
+
+        std::vector<ExpDiag> hd((size_t)niter * n);
+        std::vector<IterDiag> hi(niter);
+        HIPCHK(hipMemcpy(hd.data(), diag_hist.p, hd.size() * sizeof(ExpDiag), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hi.data(), idiag_hist.p, hi.size() * sizeof(IterDiag), hipMemcpyDeviceToHost));
+        for (int it = 0; it < niter; ++it) {
+            uint64_t nn = 0, nt = 0;
+            double mn = INFINITY;
+            for (int k = 0; k < n; ++k) {
+                const ExpDiag &d = hd[(size_t)it * n + k];
+                const double m = d.min_key == ~0ull ? INFINITY : dkey_inv(d.min_key);
+                if (rows) {
+                    updc_freeze_row &r = rows[(size_t)it * n + k];
+                    r.iteration = it + 1;
+                    r.exposure = k + 1;
+                    r.n_new_frozen = d.n_new;
+                    r.n_total_frozen = d.n_total;
+                    r.min_raw_mu = m;
+                }
+                nn += d.n_new;
+                nt += d.n_total;
+                mn = std::min(mn, m);
+            }
+            if (info) {
+                info[it].iteration = it + 1;
+                info[it].signed_sum = hi[it].signed_sum;
+                info[it].n_negative = hi[it].n_negative;
+                info[it].n_new_frozen = nn;
+                info[it].n_total_frozen = nt;
+                info[it].min_raw_mu = mn;
+                info[it].ms_total = total_ms / std::max(1, niter);
+            }
+            const unsigned int eb = hi[it].err;
+            if (eb) {
+                char msg[256];
+                const char *what = (eb & UPDC_E_MU) ? "non-finite forward mean"
+                                   : (eb & UPDC_E_RATIO) ? "non-finite y/raw_mu ratio"
+                                   : (eb & UPDC_E_FACTOR) ? "non-finite multiplicative correction"
+                                   : (eb & UPDC_E_NEGCORR) ? "substantially negative multiplicative correction"
+                                                           : "non-finite signed UPDC estimate";
+                std::snprintf(msg, sizeof msg, "%s (iteration %d, index %llu)", what, it + 1,
+                              (unsigned long long)hi[it].err_index);
+                throw Error(UPDC_ERR_NUMERIC, msg);
+            }
+        }
+    }
+
+    // ---- transfers
+    template <class T> void to_host_f64(const T *d, size_t cnt, double *out, Ctx &c) {
+        if (sizeof(T) == 8) {
+            HIPCHK(hipMemcpy(out, d, cnt * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<T> tmp(cnt);
+            HIPCHK(hipMemcpy(tmp.data(), d, cnt * sizeof(T), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < cnt; ++i) out[i] = (double)tmp[i];
+        }
+    }
+    template <class T> void from_host_f64(const double *in, size_t cnt, T *d, Ctx &c) {
+        if (sizeof(T) == 8) {
+            HIPCHK(hipMemcpy(d, in, cnt * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<T> tmp(cnt);
+            for (size_t i = 0; i < cnt; ++i) tmp[i] = (T)in[i];
+            HIPCHK(hipMemcpy(d, tmp.data(), cnt * sizeof(T), hipMemcpyHostToDevice));
+        }
+    }
+    void get_estimate(Ctx &c, double *d, float *f) override {
+        if (d) to_host_f64(x.p, c.nout(), d, c);
+        if (f) {
+            if (sizeof(TF) == 4) HIPCHK(hipMemcpy(f, x.p, c.nout() * 4, hipMemcpyDeviceToHost));
+            else {
+                std::vector<double> tmp(c.nout());
+                to_host_f64(x.p, c.nout(), tmp.data(), c);
+                for (size_t i = 0; i < tmp.size(); ++i) f[i] = (float)tmp[i];
+            }
+        }
+    }
+    void set_estimate(Ctx &c, const double *d) override { from_host_f64(d, c.nout(), x.p, c); }
+    void get_wht(Ctx &c, float *out) override {
+        std::vector<double> tmp(c.nout());
+        to_host_f64(cov.p, c.nout(), tmp.data(), c);
+        // coverage * weight_scale in f32 (updc_core.c:1482)
+        for (size_t i = 0; i < tmp.size(); ++i) out[i] = (float)tmp[i] * c.weight_scale;
+    }
+    void get_denominator(Ctx &c, double *out) override { to_host_f64(D.p, c.nout(), out, c); }
+    void get_initial(Ctx &c, double *est, double *covo) override {
+        if (est) to_host_f64(x.p, c.nout(), est, c);
+        if (covo) to_host_f64(cov.p, c.nout(), covo, c);
+    }
+    void get_psf_spectrum(Ctx &c, int k, double *out) override {
+        const FftShape &s = fft.s;
+        std::vector<double> row((size_t)s.NCP * 2);
+        for (int r = 0; r < s.FH; ++r) {
+            to_host_f64((const TF *)(psf_spec.p + ((size_t)k * s.FH + r) * s.NCP), (size_t)s.NCP * 2, row.data(), c);
+            std::memcpy(out + (size_t)r * s.NC * 2, row.data(), (size_t)s.NC * 2 * sizeof(double));
+        }
+    }
+    void stage_forward(Ctx &c, const double *xin, double *bl) override {
+        const int n = N(c);
+        const FftShape &s = fft.s;
+        const size_t nout = c.nout();
+        DevBuf<TF> xs;
+        xs.alloc(nout);
+        from_host_f64(xin, nout, xs.p, c);
+        const size_t kstride = (size_t)c.H * s.NCP;
+        HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, (size_t)s.FH * s.NCP,
+                                        c.H, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        to_host_f64(blur.p, (size_t)n * nout, bl, c);
+    }
+    void stage_model(Ctx &c, const double *xin, double *pred) override {
+        const int n = N(c);
+        const FftShape &s = fft.s;
+        const size_t nout = c.nout();
+        DevBuf<TF> xs;
+        xs.alloc(nout);
+        from_host_f64(xin, nout, xs.p, c);
+        const size_t kstride = (size_t)c.H * s.NCP;
+        HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, (size_t)s.FH * s.NCP,
+                                        c.H, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
+        HIPCHK(predict<TF>(d_ex.p, n, max_nin, blur.p, nout, wr.p, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        to_host_f64(wr.p, c.npix(), pred, c);
+    }
+    void stage_detector(Ctx &c, int k, const double *bl, double *wro, double *scat, double *stats) override {
+        const size_t nout = c.nout();
+        HostExp &e = *c.ex[k];
+        from_host_f64(bl, nout, blur.p + (size_t)k * nout, c);
+        HIPCHK(init_diag(diag_cur.p, N(c), idiag_cur.p, 1, c.st));
+        HIPCHK((gather_freeze<TF, TA>(d_ex.p, k, k + 1, (uint32_t)e.nin, blur.p, nout, sci.p, w.p, frozen.p, wr.p,
+                                      diag_cur.p, idiag_cur.p, c.st)));
+        DevBuf<TF> out;
+        out.alloc(nout);
+        HIPCHK(csc_apply<TF>(d_ex.p + k, wr.p, nout, out.p, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        if (wro) to_host_f64(wr.p + e.pix, e.nin, wro, c);
+        if (scat) to_host_f64(out.p, nout, scat, c);
+        ExpDiag d;
+        IterDiag id;
+        HIPCHK(hipMemcpy(&d, diag_cur.p + k, sizeof d, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&id, idiag_cur.p, sizeof id, hipMemcpyDeviceToHost));
+        if (stats) {
+            stats[0] = (double)d.n_new;
+            stats[1] = (double)d.n_total;
+            stats[2] = d.min_key == ~0ull ? INFINITY : dkey_inv(d.min_key);
+        }
+        if (id.err) throw Error(UPDC_ERR_NUMERIC, (id.err & UPDC_E_MU) ? "non-finite forward mean" : "non-finite y/raw_mu ratio");
+    }
+    void stage_adjoint(Ctx &c, const double *v, double *out) override {
+        const int n = N(c);
+        const FftShape &s = fft.s;
+        const size_t kstride = (size_t)c.H * s.NCP;
+        from_host_f64(v, c.npix(), wr.p, c);
+        HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, n, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, (size_t)s.FH * s.NCP,
+                                        c.H, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        to_host_f64(C.p, c.nout(), out, c);
+    }
+    void stage_scatter(Ctx &c, int k, const double *v, double *out) override {
+        HostExp &e = *c.ex[k];
+        from_host_f64(v, e.nin, wr.p + e.pix, c);
+        DevBuf<TF> o;
+        o.alloc(c.nout());
+        HIPCHK(csc_apply<TF>(d_ex.p + k, wr.p, c.nout(), o.p, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        to_host_f64(o.p, c.nout(), out, c);
+    }
+    void fft_r2c(Ctx &c, const double *img, double *spec) override {
+        const FftShape &s = fft.s;
+        DevBuf<TF> im;
+        im.alloc((size_t)s.FW * s.FH);
+        from_host_f64(img, (size_t)s.FW * s.FH, im.p, c);
+        DevBuf<Cx<TF>> sp;
+        sp.alloc((size_t)s.FH * s.NCP);
+        HIPCHK(fft.row_r2c(ROW_PLAIN, im.p, 0, s.FW, s.FW, nullptr, nullptr, 0, sp.p, 0, s.FH, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_FWD, sp.p, 0, sp.p, 0, nullptr, 0, s.FH, s.FH, 1, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        std::vector<double> row((size_t)s.NCP * 2);
+        for (int r = 0; r < s.FH; ++r) {
+            to_host_f64((const TF *)(sp.p + (size_t)r * s.NCP), (size_t)s.NCP * 2, row.data(), c);
+            std::memcpy(spec + (size_t)r * s.NC * 2, row.data(), (size_t)s.NC * 2 * sizeof(double));
+        }
+    }
+    void fft_c2r(Ctx &c, const double *spec, double *img) override {
+        const FftShape &s = fft.s;
+        std::vector<double> padded((size_t)s.FH * s.NCP * 2, 0.0);
+        for (int r = 0; r < s.FH; ++r)
+            std::memcpy(padded.data() + (size_t)r * s.NCP * 2, spec + (size_t)r * s.NC * 2, (size_t)s.NC * 2 * 8);
+        DevBuf<Cx<TF>> sp;
+        sp.alloc((size_t)s.FH * s.NCP);
+        from_host_f64(padded.data(), padded.size(), (TF *)sp.p, c);
+        HIPCHK(fft.template colpass<TA>(COL_INV, sp.p, 0, sp.p, 0, nullptr, 0, s.FH, s.FH, 1, c.st));
+        DevBuf<TF> im;
+        im.alloc((size_t)s.FW * s.FH);
+        HIPCHK(fft.row_c2r(sp.p, 0, im.p, 0, s.FW, s.FW, (TF)1, s.FH, 1, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        to_host_f64(im.p, (size_t)s.FW * s.FH, img, c);
+    }
+    void get_frozen(Ctx &c, int k, uint8_t *out) override {
+        HIPCHK(hipMemcpy(out, frozen.p + c.ex[k]->pix, c.ex[k]->nin, hipMemcpyDeviceToHost));
+    }
+    void reset_frozen(Ctx &c) override {
+        HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, c.npix(), c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+    }
+};
+
+// ------------------------------------------------------------------ exposures
+static void check_dims(Ctx &c, int iw, int ih, int pw, int ph) {
+    if (iw < 1 || ih < 1) throw Error(UPDC_ERR_ARG, "invalid input dimensions");
+    if (pw < 1 || ph < 1) throw Error(UPDC_ERR_ARG, "invalid PSF dimensions in manifest");
+    if ((size_t)iw * ih > 0xffffffffull) throw Error(UPDC_ERR_ARG, "detector exceeds 32-bit pixel index capacity");
+    if (c.prepared) throw Error(UPDC_ERR_STATE, "exposures must be added before updc_prepare");
+}
+
+static HostExp &new_exposure(Ctx &c, const float *sci, const float *wht, int iw, int ih, const float *psf, int pw,
+                             int ph, int pt, double bg, double ped) {
+    if (!std::isfinite(bg) || !std::isfinite(ped) || ped < 0.0) throw Error(UPDC_ERR_ARG, "invalid background/pedestal in manifest");
+    auto e = std::make_unique<HostExp>();
+    e->iw = iw;
+    e->ih = ih;
+    e->nin = (size_t)iw * ih;
+    e->pix = c.npix();
+    e->sci.assign(sci, sci + e->nin);
+    e->wht.assign(wht, wht + e->nin);
+    e->psf.assign(psf, psf + (size_t)pw * ph);
+    e->pw = pw;
+    e->ph = ph;
+    e->psf_t = pt ? 1 : 0;
+    e->background = bg;
+    e->pedestal = ped;
+    e->background_f = (float)bg;
+    e->pedestal_f = (float)ped;
+    e->mean_offset_f = e->background_f + e->pedestal_f;
+    if (!std::isfinite(e->mean_offset_f) || e->pedestal_f < 0.0f)
+        throw Error(UPDC_ERR_ARG, "invalid background/pedestal in manifest");
+    c.ex.push_back(std::move(e));
+    return *c.ex.back();
+}
+
+static void validate_linkmap(Ctx &c, const uint8_t *img, size_t bytes, size_t nin) {
+    // updc_core.c:485-496 and :652
+    if (bytes < 64) throw Error(UPDC_ERR_FORMAT, "invalid or obsolete LinkMap cache");
+    uint32_t h[16];
+    std::memcpy(h, img, 64);
+    uint64_t hn, hl;
+    std::memcpy(&hn, img + 16, 8);
+    std::memcpy(&hl, img + 24, 8);
+    if (h[0] != UPDC_LM_MAGIC || h[1] != UPDC_LM_VERSION || h[8] != (uint32_t)c.W || h[9] != (uint32_t)c.H ||
+        h[3] != 64 || hn != nin || hl > 0xffffffffull)
+        throw Error(UPDC_ERR_FORMAT, "invalid or obsolete LinkMap cache");
+    const size_t expect = linkmap_bytes(hn, hl, h[2]);
+    if (!expect || expect != bytes) throw Error(UPDC_ERR_FORMAT, "invalid or obsolete LinkMap cache");
+    uint32_t last;
+    std::memcpy(&last, img + 64 + nin * 4, 4);
+    if (last != (uint32_t)hl) throw Error(UPDC_ERR_FORMAT, "LinkMap offset/link-count mismatch");
+}
+
+static void add_linkmap_image(Ctx &c, HostExp &e, const void *img, size_t bytes) {
+    validate_linkmap(c, (const uint8_t *)img, bytes, e.nin);
+    std::memcpy(&e.encoding, (const uint8_t *)img + 8, 4);
+    std::memcpy(&e.nlinks, (const uint8_t *)img + 24, 8);
+    e.lm_bytes = bytes;
+    e.lm.alloc(bytes);
+    HIPCHK(hipMemcpy(e.lm.p, img, bytes, hipMemcpyHostToDevice));
+    if (e.encoding != UPDC_LM_DELTA16 && e.encoding != UPDC_LM_ABS32) throw Error(UPDC_ERR_FORMAT, "invalid LinkMap encoding");
+    // the offsets are checked on the device before any kernel follows them
+    DevBuf<unsigned int> flags;
+    flags.alloc(1);
+    HIPCHK(fill<unsigned int>(flags.p, 0u, 1, c.st));
+    const ExpDev v = e.view();
+    HIPCHK(lm_validate(v.off, v.base, v.d16, v.t32, v.frac, e.encoding, (uint32_t)e.nin, e.nlinks, c.nout(), flags.p,
+                       c.st));
+    unsigned int fl = 0;
+    HIPCHK(hipMemcpyAsync(&fl, flags.p, 4, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    if (fl & 1u) throw Error(UPDC_ERR_FORMAT, "LinkMap offsets are not monotone / exceed the link count");
+    if (fl & 2u) throw Error(UPDC_ERR_FORMAT, "LinkMap target outside the target grid");
+    if (fl & 4u) throw Error(UPDC_ERR_FORMAT, "LinkMap fraction is not finite");
+}
+
+// GPU LinkMap build from a vertex lattice (updc_core.c:530-609), two passes.
+static void build_linkmap(Ctx &c, HostExp &e, const double *vert) {
+    const size_t nv = 2 * (size_t)(e.iw + 1) * (e.ih + 1);
+    DevBuf<double> dv;
+    dv.alloc(nv);
+    HIPCHK(hipMemcpyAsync(dv.p, vert, nv * 8, hipMemcpyHostToDevice, c.st));
+    DevBuf<uint32_t> cnt, first;
+    cnt.alloc(e.nin);
+    first.alloc(e.nin);
+    DevBuf<unsigned int> flags;
+    flags.alloc(1);
+    HIPCHK(fill<unsigned int>(flags.p, 0u, 1, c.st));
+    HIPCHK(lm_count(dv.p, e.iw, e.ih, c.W, c.H, cnt.p, first.p, flags.p, c.st));
+    DevBuf<uint32_t> off;
+    off.alloc(e.nin + 1);
+    DevBuf<uint8_t> scratch;
+    scratch.alloc(scan_scratch_bytes(e.nin));
+    DevBuf<unsigned long long> tot;
+    tot.alloc(1);
+    HIPCHK(scan_u32(cnt.p, off.p, e.nin, scratch.p, tot.p, c.st));
+    unsigned int fl = 0;
+    unsigned long long total = 0;
+    HIPCHK(hipMemcpyAsync(&fl, flags.p, 4, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipMemcpyAsync(&total, tot.p, 8, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    if (total >= 0xffffffffull) throw Error(UPDC_ERR_ARG, "LinkMap exceeds uint32 offset capacity");
+    e.encoding = (fl & 1u) ? UPDC_LM_ABS32 : UPDC_LM_DELTA16;
+    e.nlinks = total;
+    e.lm_bytes = linkmap_bytes(e.nin, total, e.encoding);
+    e.lm.alloc(e.lm_bytes);
+    HIPCHK(hipMemsetAsync(e.lm.p, 0, e.lm_bytes, c.st));
+    uint32_t h[16] = {0};
+    h[0] = UPDC_LM_MAGIC; h[1] = UPDC_LM_VERSION; h[2] = e.encoding; h[3] = 64;
+    const uint64_t nin64 = e.nin;
+    std::memcpy(&h[4], &nin64, 8);
+    std::memcpy(&h[6], &total, 8);
+    h[8] = (uint32_t)c.W; h[9] = (uint32_t)c.H;
+    HIPCHK(hipMemcpyAsync(e.lm.p, h, 64, hipMemcpyHostToDevice, c.st));
+    HIPCHK(hipMemcpyAsync(e.lm.p + 64, off.p, (e.nin + 1) * 4, hipMemcpyDeviceToDevice, c.st));
+    ExpDev v = e.view();
+    if (e.encoding == UPDC_LM_DELTA16)
+        HIPCHK(hipMemcpyAsync((void *)v.base, first.p, e.nin * 4, hipMemcpyDeviceToDevice, c.st));
+    HIPCHK(fill<unsigned int>(flags.p, 0u, 1, c.st));
+    HIPCHK(lm_fill(dv.p, e.iw, e.ih, c.W, c.H, v.off, v.base, e.encoding, (uint16_t *)v.d16, (uint32_t *)v.t32,
+                   (float *)v.frac, flags.p, c.st));
+    HIPCHK(hipMemcpyAsync(&fl, flags.p, 4, hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    if (fl & 2u) throw Error(UPDC_ERR_ARG, "LinkMap second-pass count overflow");
+    if (fl & 4u) throw Error(UPDC_ERR_ARG, "DELTA16 LinkMap invariant failed");
+    if (fl & 8u) throw Error(UPDC_ERR_ARG, "LinkMap second-pass count mismatch");
+}
+
+}  // namespace updc
+
+using namespace updc;
+
+struct updc_ctx : Ctx {};
+
+template <class F> static int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return UPDC_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "out of memory";
+        return UPDC_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return UPDC_ERR_ARG;
+    }
+}
+static void need(bool ok, int code, const char *msg) {
+    if (!ok) throw Error(code, msg);
+}
+
+extern "C" {
+
+const char *updc_last_error(void) { return g_last_error.c_str(); }
+
+float updc_polygon_area(const double *xy, int count) { return geo::area((const geo::P *)xy, count); }
+float updc_overlap_area(const double *quad_xy, int x, int y) { return geo::overlap((const geo::P *)quad_xy, x, y); }
+
+int updc_self_test(int verbose) {
+    const double unit[8] = {1000.0, 2000.0, 1000.0, 2001.0, 1001.0, 2001.0, 1001.0, 2000.0};
+    const float a = updc_polygon_area(unit, 4);
+    if (std::fabs(a - 1.0f) > 2.0e-6f) {
+        std::fprintf(stderr, "self-test polygon area failed: %.9g\n", (double)a);
+        return 1;
+    }
+    const double half[8] = {-0.5, -0.5, -0.5, 0.5, 0.0, 0.5, 0.0, -0.5};
+    const float o = updc_overlap_area(half, 0, 0);
+    if (std::fabs(o - 0.5f) > 2.0e-6f) {
+        std::fprintf(stderr, "self-test overlap failed: %.9g\n", (double)o);
+        return 1;
+    }
+    struct Hdr { uint32_t a, b, c, d; uint64_t e, f; uint32_t g, h; uint32_t r[6]; };
+    struct Tmp { uint32_t t; float f; };
+    if (sizeof(Hdr) != 64 || sizeof(Tmp) != 8) {
+        std::fprintf(stderr, "self-test LinkMap ABI failed\n");
+        return 1;
+    }
+    if (verbose) std::printf("UPDC C core self-test: OK\n");
+    return 0;
+}
+
+int updc_ctx_create(updc_ctx **out, int width, int height, int fft_width, int fft_height, int precision,
+                    unsigned flags, int device) {
+    return guarded([&] {
+        need(out != nullptr, UPDC_ERR_ARG, "null context pointer");
+        *out = nullptr;
+        need(width >= 1 && height >= 1, UPDC_ERR_ARG, "invalid dimensions/exposures/iterations/workers/save_every");
+        need((size_t)width * height <= 0xffffffffull, UPDC_ERR_ARG, "target grid exceeds 32-bit link index capacity");
+        need(precision >= UPDC_PREC_F32 && precision <= UPDC_PREC_MIXED, UPDC_ERR_ARG, "unknown precision");
+        need(fft_width >= 0 && fft_height >= 0, UPDC_ERR_ARG, "invalid FFT shape");
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        need(device >= 0 && device < ndev, UPDC_ERR_HIP, "no such HIP device");
+        HIPCHK(hipSetDevice(device));
+        auto c = std::make_unique<updc_ctx>();
+        c->W = width;
+        c->H = height;
+        c->FW = fft_width;
+        c->FH = fft_height;
+        c->prec = precision;
+        c->flags = flags;
+        c->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+int updc_ctx_destroy(updc_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        ctx->pipe.reset();
+        ctx->ex.clear();
+        if (ctx->st) (void)hipStreamDestroy(ctx->st);
+        delete ctx;
+    });
+}
+
+int updc_ctx_set_preprocess_workers(updc_ctx *ctx, int workers) {
+    return guarded([&] {
+        need(ctx && workers >= 1, UPDC_ERR_ARG, "invalid worker count");
+        ctx->workers = workers;
+    });
+}
+
+int updc_add_exposure(updc_ctx *ctx, const float *sci, const float *wht, int iw, int ih, const void *linkmap,
+                      size_t linkmap_bytes, const float *psf, int pw, int ph, int psf_t, double bg, double ped) {
+    return guarded([&] {
+        need(ctx && sci && wht && psf && linkmap, UPDC_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(ctx->device));
+        check_dims(*ctx, iw, ih, pw, ph);
+        HostExp &e = new_exposure(*ctx, sci, wht, iw, ih, psf, pw, ph, psf_t, bg, ped);
+        try {
+            add_linkmap_image(*ctx, e, linkmap, linkmap_bytes);
+        } catch (...) {
+            ctx->ex.pop_back();
+            throw;
+        }
+    });
+}
+
+int updc_add_exposure_vertices(updc_ctx *ctx, const float *sci, const float *wht, int iw, int ih,
+                               const double *vert, const float *psf, int pw, int ph, int psf_t, double bg,
+                               double ped) {
+    return guarded([&] {
+        need(ctx && sci && wht && psf && vert, UPDC_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(ctx->device));
+        check_dims(*ctx, iw, ih, pw, ph);
+        HostExp &e = new_exposure(*ctx, sci, wht, iw, ih, psf, pw, ph, psf_t, bg, ped);
+        try {
+            build_linkmap(*ctx, e, vert);
+        } catch (...) {
+            ctx->ex.pop_back();
+            throw;
+        }
+    });
+}
+
+int updc_linkmap_size(updc_ctx *ctx, int k, size_t *bytes) {
+    return guarded([&] {
+        need(ctx && bytes && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        *bytes = ctx->ex[k]->lm_bytes;
+    });
+}
+int updc_get_linkmap(updc_ctx *ctx, int k, void *out, size_t cap) {
+    return guarded([&] {
+        need(ctx && out && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        need(cap >= ctx->ex[k]->lm_bytes, UPDC_ERR_ARG, "buffer too small for LinkMap");
+        HIPCHK(hipSetDevice(ctx->device));
+        HIPCHK(hipMemcpy(out, ctx->ex[k]->lm.p, ctx->ex[k]->lm_bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int updc_prepare(updc_ctx *ctx) {
+    return guarded([&] {
+        need(ctx != nullptr, UPDC_ERR_ARG, "null context");
+        need(!ctx->prepared, UPDC_ERR_STATE, "context already prepared");
+        need(!ctx->ex.empty(), UPDC_ERR_ARG, "no exposures");
+        HIPCHK(hipSetDevice(ctx->device));
+        int pad_w = 0, pad_h = 0;
+        for (auto &e : ctx->ex) {
+            pad_w = std::max(pad_w, e->psf_t ? e->ph : e->pw);
+            pad_h = std::max(pad_h, e->psf_t ? e->pw : e->ph);
+        }
+        const int min_w = ctx->W + pad_w - 1, min_h = ctx->H + pad_h - 1;
+        if (ctx->FW == 0) ctx->FW = next_even_smooth(min_w);
+        if (ctx->FH == 0) ctx->FH = next_smooth(min_h);
+        need(ctx->FW >= min_w && ctx->FH >= min_h, UPDC_ERR_ARG, "FFT shape smaller than target + PSF - 1");
+        need(!(ctx->FW & 1) && seven_smooth(ctx->FW) && seven_smooth(ctx->FH), UPDC_ERR_ARG,
+             "FFT shape must be 7-smooth with an even width");
+        if (ctx->prec == UPDC_PREC_F32) ctx->pipe = std::make_unique<Pipe<float, float>>();
+        else if (ctx->prec == UPDC_PREC_F64) ctx->pipe = std::make_unique<Pipe<double, double>>();
+        else ctx->pipe = std::make_unique<Pipe<float, double>>();
+        try {
+            ctx->pipe->prepare(*ctx);
+        } catch (...) {
+            ctx->pipe.reset();
+            throw;
+        }
+        // host copies are no longer needed
+        for (auto &e : ctx->ex) {
+            std::vector<float>().swap(e->sci);
+            std::vector<float>().swap(e->wht);
+        }
+        ctx->prepared = true;
+    });
+}
+
+#define NEED_PREPARED(ctx) need((ctx) && (ctx)->prepared, UPDC_ERR_STATE, "context not prepared")
+
+int updc_iterate(updc_ctx *ctx, int n, updc_freeze_row *rows, updc_iter_info *info) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(n >= 1, UPDC_ERR_ARG, "invalid iteration count");
+        HIPCHK(hipSetDevice(ctx->device));
+        ctx->pipe->iterate(*ctx, n, rows, info);
+    });
+}
+int updc_get_estimate(updc_ctx *ctx, float *out) {
+    return guarded([&] { NEED_PREPARED(ctx); need(out, UPDC_ERR_ARG, "null"); ctx->pipe->get_estimate(*ctx, nullptr, out); });
+}
+int updc_get_estimate_f64(updc_ctx *ctx, double *out) {
+    return guarded([&] { NEED_PREPARED(ctx); need(out, UPDC_ERR_ARG, "null"); ctx->pipe->get_estimate(*ctx, out, nullptr); });
+}
+int updc_set_estimate_f64(updc_ctx *ctx, const double *in) {
+    return guarded([&] { NEED_PREPARED(ctx); need(in, UPDC_ERR_ARG, "null"); ctx->pipe->set_estimate(*ctx, in); });
+}
+int updc_get_wht(updc_ctx *ctx, float *out) {
+    return guarded([&] { NEED_PREPARED(ctx); need(out, UPDC_ERR_ARG, "null"); ctx->pipe->get_wht(*ctx, out); });
+}
+int updc_get_denominator_f64(updc_ctx *ctx, double *out) {
+    return guarded([&] { NEED_PREPARED(ctx); need(out, UPDC_ERR_ARG, "null"); ctx->pipe->get_denominator(*ctx, out); });
+}
+int updc_get_initial_f64(updc_ctx *ctx, double *est, double *cov) {
+    return guarded([&] { NEED_PREPARED(ctx); ctx->pipe->get_initial(*ctx, est, cov); });
+}
+int updc_get_psf_spectrum_f64(updc_ctx *ctx, int k, double *out) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(out && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        ctx->pipe->get_psf_spectrum(*ctx, k, out);
+    });
+}
+int updc_get_scalars(updc_ctx *ctx, double *ws, double *ms, int *fw, int *fh, uint64_t *masked) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        if (ws) *ws = ctx->weight_scale;
+        if (ms) *ms = ctx->pipe->max_sens;
+        if (fw) *fw = ctx->FW;
+        if (fh) *fh = ctx->FH;
+        if (masked) *masked = ctx->masked;
+    });
+}
+int updc_get_frozen(updc_ctx *ctx, int k, uint8_t *out) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(out && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        ctx->pipe->get_frozen(*ctx, k, out);
+    });
+}
+int updc_reset_frozen(updc_ctx *ctx) {
+    return guarded([&] { NEED_PREPARED(ctx); ctx->pipe->reset_frozen(*ctx); });
+}
+int updc_stage_forward(updc_ctx *ctx, const double *x, double *blur) {
+    return guarded([&] { NEED_PREPARED(ctx); need(x && blur, UPDC_ERR_ARG, "null"); ctx->pipe->stage_forward(*ctx, x, blur); });
+}
+int updc_stage_model(updc_ctx *ctx, const double *x, double *pred) {
+    return guarded([&] { NEED_PREPARED(ctx); need(x && pred, UPDC_ERR_ARG, "null"); ctx->pipe->stage_model(*ctx, x, pred); });
+}
+int updc_stage_detector(updc_ctx *ctx, int k, const double *blur, double *wr, double *scatter, double *stats) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(blur && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        ctx->pipe->stage_detector(*ctx, k, blur, wr, scatter, stats);
+    });
+}
+int updc_stage_adjoint(updc_ctx *ctx, const double *v, double *out) {
+    return guarded([&] { NEED_PREPARED(ctx); need(v && out, UPDC_ERR_ARG, "null"); ctx->pipe->stage_adjoint(*ctx, v, out); });
+}
+int updc_stage_scatter(updc_ctx *ctx, int k, const double *v, double *out) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(v && out && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        ctx->pipe->stage_scatter(*ctx, k, v, out);
+    });
+}
+int updc_fft_r2c_f64(updc_ctx *ctx, const double *image, double *spectrum) {
+    return guarded([&] { NEED_PREPARED(ctx); need(image && spectrum, UPDC_ERR_ARG, "null"); ctx->pipe->fft_r2c(*ctx, image, spectrum); });
+}
+int updc_fft_c2r_f64(updc_ctx *ctx, const double *spectrum, double *image) {
+    return guarded([&] { NEED_PREPARED(ctx); need(image && spectrum, UPDC_ERR_ARG, "null"); ctx->pipe->fft_c2r(*ctx, spectrum, image); });
+}
+int updc_get_stage_times(updc_ctx *ctx, char names[][32], double *ms, int *launches, int *count) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(count != nullptr, UPDC_ERR_ARG, "null");
+        PipeBase &p = *ctx->pipe;
+        const int have = (int)p.stage_ms.size();
+        const int m = std::min(*count, have);
+        for (int i = 0; i < m; ++i) {
+            const std::string nm = i < (int)p.stage_names.size() ? p.stage_names[i] : std::string("iteration");
+            if (names) {
+                std::strncpy(names[i], nm.c_str(), 31);
+                names[i][31] = 0;
+            }
+            if (ms) ms[i] = p.stage_ms[i];
+            if (launches) launches[i] = p.stage_n[i];
+        }
+        *count = m;
+    });
+}
+
+}  // extern "C"
