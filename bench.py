@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark of the UPDC v17 reconstruction iteration on one MI355X.
+
+Workload (default): 8 exposures of 2048^2 -> 4096^2 target, Airy+wing PSFs of
+1431^2, FFT 5600^2, 40 iterations (synthetic, seeded; see updc_amd/synth.py).
+A "step" is one reconstruction iteration over all exposures.
+
+Reported line: iterations/s of the FP32/batched form (FP32 FFTs, PSF and S_k
+products, FP64 spectrum accumulation over exposures, denominator, raw_mu test
+and ratio; all exposures batched in every launch; each iteration replayed as
+a hipGraph), next to the plain FP64 GPU form and the error of the FP32/batched
+estimate against the FP64 one after the same number of iterations.
+
+Usage: python bench.py [--gpus 1] [--steps 40] [--warmup 2] [--config 2|1]
+                       [--no-cpu] [--no-f64] [--profile]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "RL iters/s, 8x2048² exps->4096² grid: FP32/batched vs FP64, rel err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_dataset(config: int):
+    from updc_amd import synth
+    from updc_amd.solver import Context, Exposure
+    t0 = time.time()
+    ds = synth.config2() if config == 2 else synth.config1()
+    log(f"synthetic {ds.name}: truth/PSFs/geometry in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    n = len(ds.geoms)
+    with Context(ds.W, ds.H, "f64") as mc:
+        for k, g in enumerate(ds.geoms):
+            z = np.zeros((g.ih, g.iw), np.float32)
+            mc.add_exposure(Exposure(z, z + 1.0, ds.psfs[k], 0.0, 0.0, vertices=ds.vertices(k)))
+        mc.prepare()
+        model = mc.model(ds.truth)
+        linkmaps = [mc.linkmap(k) for k in range(n)]
+    synth.observe(ds, [m + b for m, b in zip(model, ds.background)], config_seed=config)
+    ds.linkmaps = linkmaps
+    log(f"noise-free model on GPU + observation in {time.time() - t0:.1f}s; pedestals {ds.pedestal[:2]}...")
+    return ds
+
+
+def solve_ctx(ds, precision: str, graph: bool):
+    from updc_amd.solver import Context, Exposure
+    c = Context(ds.W, ds.H, precision, graph=graph)
+    for k in range(len(ds.geoms)):
+        c.add_exposure(Exposure(ds.sci[k], ds.wht[k], ds.psfs[k], ds.background[k], ds.pedestal[k],
+                                linkmap=ds.linkmaps[k]))
+    t0 = time.time()
+    c.prepare()
+    log(f"{precision}{' graph' if graph else ''}: prepare {time.time() - t0:.2f}s {c.scalars()}")
+    return c
+
+
+def timed_run(c, steps: int, warmup: int):
+    """W untimed iterations (graph capture happens there), reset to x0 and the
+    zero FreezeMu state, then exactly K timed iterations."""
+    x0, _ = c.initial()
+    if warmup:
+        c.iterate(warmup)
+    c.set_estimate(x0)
+    c.reset_frozen()
+    t0 = time.perf_counter()
+    rows, info = c.iterate(steps)    # synchronises before returning
+    dt = time.perf_counter() - t0
+    return dt, rows, info
+
+
+def stage_profile(c, iters: int = 3) -> dict:
+    """Per-kernel average device times (HIP events on the library stream),
+    from an eager (non-graph) run on a scratch copy of the state."""
+    x = c.estimate(f64=True)
+    c.set_graph(False)
+    c.iterate(iters)
+    t = c.stage_times()
+    c.set_graph(True)
+    c.set_estimate(x)
+    return t
+
+
+def cpu_baseline(ds, fft_shape) -> dict:
+    """The oracle's float32 restatement (numpy pocketfft + C detector loops,
+    single thread) timed on this host for one iteration with N=1 and N=2
+    exposures of the same workload; t(N) = a + b N is extrapolated to N."""
+    from oracle import oracle as O
+    times = {}
+    for n in (1, 2):
+        ex = [dict(sci=ds.sci[k], wht=ds.wht[k], psf=ds.psfs[k], background=ds.background[k],
+                   pedestal=ds.pedestal[k], linkmap=ds.linkmaps[k]) for k in range(n)]
+        s = O.RefSolver(ds.W, ds.H, ex, "f32", fft_shape=fft_shape)
+        t0 = time.perf_counter()
+        s.iterate(1)
+        times[n] = time.perf_counter() - t0
+        log(f"cpu oracle: N={n} one iteration {times[n]:.2f}s")
+    b = times[2] - times[1]
+    a = times[1] - b
+    N = len(ds.geoms)
+    t = a + b * N
+    return dict(value=1.0 / t, unit="iters/s", cores=1, kind="port",
+                sample=f"1 iteration of the oracle's f32 path with N=1 and N=2 exposures of this workload "
+                       f"(FFT {fft_shape[0]}x{fft_shape[1]}), t(N)=a+bN extrapolated to N={N}: "
+                       f"t1={times[1]:.2f}s t2={times[2]:.2f}s -> {t:.2f}s/iter; numpy pocketfft in place of FFTW3f")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="only the FP32/batched timed run (for rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: the multi-GPU form (exposures sharded across GPUs, one all-reduce "
+              "of the correction image per iteration) is a design, not a run; use --gpus 1", file=sys.stderr)
+        return 2
+    from updc_amd import metrics
+    ds = make_dataset(args.config)
+    K, Wm = args.steps, args.warmup
+
+    fast = solve_ctx(ds, "mixed", graph=True)
+    sc = fast.scalars()
+    FW, FH = sc["fft_width"], sc["fft_height"]
+    dt32, rows32, info32 = timed_run(fast, K, Wm)
+    its32 = K / dt32
+    log(f"FP32/batched (mixed, graph): {K} iterations in {dt32 * 1e3:.1f} ms -> {its32:.1f} it/s")
+    x32 = fast.estimate(f64=True)
+    if args.profile:
+        print(json.dumps(dict(metric=METRIC, value=its32, unit="iters/s", steps=K, warmup=Wm)))
+        return 0
+
+    # per-kernel times of the FP32/batched form (eager launches, HIP events)
+    prof = stage_profile(fast)
+    infos = [fast.exposure_info(k) for k in range(len(ds.geoms))]
+    fast.close()
+
+    ref32 = solve_ctx(ds, "f32", graph=True)
+    dtr, _, _ = timed_run(ref32, K, Wm)
+    xr32 = ref32.estimate(f64=True)
+    ref32.close()
+    log(f"reference-precision f32 (graph): {K / dtr:.1f} it/s")
+
+    its64 = None
+    err = None
+    errr = None
+    if not args.no_f64:
+        slow = solve_ctx(ds, "f64", graph=False)
+        dt64, rows64, _ = timed_run(slow, K, Wm)
+        its64 = K / dt64
+        x64 = slow.estimate(f64=True)
+        slow.close()
+        log(f"plain FP64: {K} iterations in {dt64 * 1e3:.1f} ms -> {its64:.1f} it/s")
+        err = metrics.stats(x64, x32)
+        errr = metrics.stats(x64, xr32)
+        frz_mismatch = sum(1 for a, b in zip(rows32, rows64) if a[2:4] != b[2:4])
+        log(f"FP32/batched vs FP64 after {K}: {err}; freeze-row mismatches {frz_mismatch}")
+        log(f"reference f32 vs FP64 after {K}: norm_linf={errr['norm_linf']:.3e} rel_rms={errr['rel_rms']:.3e}")
+
+    # roofline of the dominant kernel
+    e, c, ed = 4, 8, 8
+    sb = metrics.stage_bytes(ds.W, ds.H, FW, FH, [i["nin"] for i in infos], [i["nlinks"] for i in infos],
+                             [i["ncsc"] for i in infos], [i["linkmap_bytes"] for i in infos], e, c, ed)
+    kern = {k: v for k, v in prof.items() if k in sb}
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms = kern[dom][0]
+    achieved = sb[dom] / (ms * 1e-3) / 1e9
+    per_stage = {k: dict(ms=round(v[0], 4), GBps=round(sb[k] / (v[0] * 1e-3) / 1e9, 1)) for k, v in kern.items()}
+    log(f"per-kernel: {json.dumps(per_stage)}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(ds, (FW, FH))
+        except Exception as ex:  # the baseline is informative; the GPU line stands without it
+            log(f"cpu baseline failed: {ex!r}")
+    line = {
+        "metric": METRIC, "value": round(its32, 3), "unit": "iters/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": round(dt32 * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 accumulation/denominator/ratio)", "data": "synthetic",
+        "config": {"workload": f"{ds.name}: {len(ds.geoms)}x{ds.geoms[0].iw}^2 exposures -> {ds.W}^2 target, "
+                               f"PSF {ds.psfs[0].shape[0]}^2, FFT {FW}x{FH}, {K} iterations",
+                   "form": "FP32/batched (all exposures per launch, hipGraph per iteration)",
+                   "fp64_iters_per_s": None if its64 is None else round(its64, 3),
+                   "fp32_over_fp64": None if its64 is None else round(its32 / its64, 3),
+                   "ref_f32_iters_per_s": round(K / dtr, 3),
+                   "rel_err_vs_fp64": None if err is None else {k: err[k] for k in (
+                       "norm_linf", "rel_rms", "max_rel_masked", "max_abs", "rms")},
+                   "ref_f32_rel_err_vs_fp64": None if errr is None else {k: errr[k] for k in ("norm_linf", "rel_rms")},
+                   "per_kernel": per_stage},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "algorithmic_bytes_per_launch": sb[dom], "avg_launch_ms": round(ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -116,6 +116,9 @@ typedef struct {
 int updc_ctx_create(updc_ctx **out, int width, int height, int fft_width, int fft_height, int precision,
                     unsigned flags, int device);
 int updc_ctx_destroy(updc_ctx *ctx);
+/* Replace the UPDC_FLAG_* set (e.g. switch hipGraph replay on or off between
+ * updc_iterate calls; per-kernel times are recorded for eager launches only). */
+int updc_ctx_set_flags(updc_ctx *ctx, unsigned flags);
 
 /* Drizzle worker topology of the initial estimate (updc_core.c:917-973):
  * 1 = the serial summation order, n > 1 = per-worker partial sums combined in
@@ -160,6 +163,10 @@ int updc_get_psf_spectrum_f64(updc_ctx *ctx, int k, double *out); /* FH x (FW/2+
 int updc_get_scalars(updc_ctx *ctx, double *weight_scale, double *max_sensitivity, int *fft_width,
                      int *fft_height, uint64_t *masked_pixels);
 int updc_get_frozen(updc_ctx *ctx, int k, uint8_t *out);
+/* Sizes of exposure k: detector pixels, LinkMap links and encoding, and the
+ * links of its CSC transpose (pixels with w > 0 only; valid after prepare). */
+int updc_get_exposure_info(updc_ctx *ctx, int k, uint64_t *nin, uint64_t *nlinks, uint32_t *encoding,
+                           uint64_t *ncsc);
 
 /* Stage entry points (parity tests, autograd).  All host buffers, f64 at the
  * boundary, computed in the context's precision on the GPU. */

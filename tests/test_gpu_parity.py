@@ -143,35 +143,87 @@ def test_detector_stage_bitexact_f32(small_ds):
 
 def test_psf_spectrum_and_denominator(small_ds):
     print("case: PSF spectra and denominator vs oracle", flush=True)
-    for prec, tol in (("f32", 5e-6), ("f64", 1e-12), ("mixed", 1e-12)):
+    # mixed: FP64 transforms on weights normalised in FP32 (the reference's
+    # normalisation), so its denominator carries the f32 weight rounding
+    for prec, tol in (("f32", 5e-6), ("f64", 1e-12), ("mixed", 2e-7)):
         ref = _oracle(small_ds, "f64" if prec != "f32" else "f32")
         c = _ctx(small_ds, prec)
         c.prepare()
         for k in range(len(small_ds.geoms)):
             P = c.psf_spectrum(k)
             Pr = ref.ex[k]["P"]
-            assert np.max(np.abs(P - Pr)) < (1e-6 if prec == "f32" else 1e-13) * np.max(np.abs(Pr))
+            # the mixed tier keeps FP32 PSF spectra by design
+            assert np.max(np.abs(P - Pr)) < (1e-6 if prec != "f64" else 1e-13) * np.max(np.abs(Pr))
         D = c.denominator()
         err = np.max(np.abs(D - ref.D)) / np.max(np.abs(ref.D))
         assert err < tol, (prec, err)
         c.close()
 
 
-@pytest.mark.parametrize("prec,ref_prec,tol", [("f32", "f32", 2e-5), ("f64", "f64", 1e-11), ("mixed", "f64", 2e-5)])
-def test_iterations_vs_oracle(small_ds, prec, ref_prec, tol):
+@pytest.fixture(scope="module")
+def oracle_runs(small_ds):
+    """FP64 oracle and the reference's FP32 algorithm (oracle f32), 5 iterations."""
     from oracle import oracle as O
-    print(f"case: 5 iterations {prec} vs oracle {ref_prec}", flush=True)
-    ref = _oracle(small_ds, ref_prec)
+    r64 = _oracle(small_ds, "f64")
+    r32 = _oracle(small_ds, "f32")
+    rows64, _ = r64.iterate(5)
+    rows32, _ = r32.iterate(5)
+    base = O.compare_stats(r64.x, r32.x)["norm_linf"]
+    return r64.x.copy(), rows64, base
+
+
+@pytest.mark.parametrize("prec,factor", [("f64", None), ("f32", 3.0), ("mixed", 2.0)])
+def test_iterations_vs_oracle(small_ds, oracle_runs, prec, factor):
+    """5 iterations against the FP64 oracle.  f64: within 1e-11.  f32/mixed:
+    within `factor` times the distance of the reference's own FP32 algorithm
+    (the f32 oracle) from FP64 -- i.e. as accurate as the reference's float32
+    arithmetic allows.  FreezeMu rows must agree exactly."""
+    from oracle import oracle as O
+    x64, rows64, base = oracle_runs
+    print(f"case: 5 iterations {prec} vs oracle f64 (oracle f32 distance {base:.3e})", flush=True)
     c = _ctx(small_ds, prec)
     c.prepare()
-    rows_g, info_g = c.iterate(5)
-    rows_o, info_o = ref.iterate(5)
-    st = O.compare_stats(ref.x, c.estimate(f64=True))
+    rows_g, _ = c.iterate(5)
+    st = O.compare_stats(x64, c.estimate(f64=True))
+    tol = 1e-11 if factor is None else factor * base + 1e-6
+    print(f"  {prec}: norm_linf {st['norm_linf']:.3e} (tol {tol:.3e})", flush=True)
     assert st["norm_linf"] < tol, st
-    # FreezeMu bookkeeping: same rows (counts) and monotone totals
-    for rg, ro in zip(rows_g, rows_o):
+    for rg, ro in zip(rows_g, rows64):
         assert rg[:4] == ro[:4]
     c.close()
+
+
+def test_stage_accuracy_vs_f64_oracle(small_ds):
+    """Forward blur, adjoint and denominator of every tier against the FP64
+    oracle (errors printed for the record, loose bounds asserted)."""
+    print("case: stage accuracy vs f64 oracle", flush=True)
+    ref = _oracle(small_ds, "f64")
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=sum(g.iw * g.ih for g in small_ds.geoms))
+    x = ref.x
+    blur_o = ref.blur(x)
+    adj_o = np.zeros((small_ds.H, small_ds.W))
+    pos = 0
+    acc = 0
+    for k, e in enumerate(ref.ex):
+        n = e["s"].size
+        vk = np.where(e["w"] > 0, v[pos:pos + n], 0.0)
+        acc = acc + ref._rfft(ref._scatter64(e, vk)) * np.conj(e["P"])
+        pos += n
+    adj_o = ref._irfft_crop(acc)
+    for prec, tol in (("f32", 3e-6), ("mixed", 3e-6), ("f64", 1e-12)):
+        c = _ctx(small_ds, prec)
+        c.prepare()
+        bl = c.forward(x)
+        eb = max(np.max(np.abs(bl[k] - blur_o[k])) for k in range(len(blur_o))) / np.max(np.abs(blur_o))
+        vm = np.concatenate([np.where(ref.ex[k]["w"] > 0, v[sum(g.iw * g.ih for g in small_ds.geoms[:k]):][:ref.ex[k]["s"].size], 0.0)
+                             for k in range(len(ref.ex))])
+        ad = c.adjoint(vm)
+        ea = np.max(np.abs(ad - adj_o)) / np.max(np.abs(adj_o))
+        ed = np.max(np.abs(c.denominator() - ref.D)) / np.max(np.abs(ref.D))
+        print(f"  {prec}: blur {eb:.3e} adjoint {ea:.3e} denominator {ed:.3e}", flush=True)
+        assert eb < tol and ea < tol, (prec, eb, ea)
+        c.close()
 
 
 def test_graph_replay_matches_eager(small_ds):

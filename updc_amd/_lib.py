@@ -49,6 +49,7 @@ SIGNATURES = {
     "updc_overlap_area": (C.c_float, [_D, C.c_int, C.c_int]),
     "updc_ctx_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_int]),
     "updc_ctx_destroy": (C.c_int, [_P]),
+    "updc_ctx_set_flags": (C.c_int, [_P, C.c_uint]),
     "updc_ctx_set_preprocess_workers": (C.c_int, [_P, C.c_int]),
     "updc_add_exposure": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, _P, C.c_size_t, _F, C.c_int, C.c_int, C.c_int,
                                     C.c_double, C.c_double]),
@@ -67,6 +68,8 @@ SIGNATURES = {
     "updc_get_psf_spectrum_f64": (C.c_int, [_P, C.c_int, _D]),
     "updc_get_scalars": (C.c_int, [_P, _D, _D, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     "updc_get_frozen": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint8)]),
+    "updc_get_exposure_info": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "updc_stage_forward": (C.c_int, [_P, _D, _D]),
     "updc_stage_model": (C.c_int, [_P, _D, _D]),
     "updc_stage_detector": (C.c_int, [_P, C.c_int, _D, _D, _D, _D]),

@@ -119,7 +119,8 @@ template <int R, bool INV, class T> __device__ __forceinline__ void bfly(Cx<T> *
 // One transform length, split into Stockham radix stages.  tw points at the
 // device table tw[m] = exp(-2 pi i m / L), m in [0, L), of the FFT's real type.
 // For stage s, ns[s] is the product of the earlier radices and mg[s] =
-// ceil(2^32 / ns[s]), so i / ns[s] = umulhi(i, mg[s]) exactly for i, ns < 2^16.
+// ceil(2^32 / ns[s]), so i / ns[s] = umulhi(i, mg[s]) exactly for i, ns < 2^16
+// (ns > 1; the first stage, ns = 1, divides trivially).
 struct FftLen {
     int L;
     int nst;
@@ -163,7 +164,8 @@ __device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int Ns, uns
         const int b = tid + r * NT;
         if (b < nb) {
             const int c = b & lm, i = b >> lg;
-            const int hi = (int)__umulhi((unsigned int)i, mg);
+            // Ns == 1 (first stage): mg = 2^32 does not fit, and i / 1 = i
+            const int hi = Ns == 1 ? i : (int)__umulhi((unsigned int)i, mg);
             const int k = i - hi * Ns;
             if (Ns > 1) {
 #pragma unroll

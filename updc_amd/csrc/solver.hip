@@ -469,6 +469,10 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, npix, c.st));
         diag_cur.alloc(n);
         idiag_cur.alloc(1);
+        // per-iteration history, sized so a captured graph survives most calls
+        hist_cap = 1024;
+        diag_hist.alloc((size_t)hist_cap * n);
+        idiag_hist.alloc(hist_cap);
         partials.alloc(update_blocks());
         negs.alloc(update_blocks());
         ticket.alloc(1);
@@ -1011,6 +1015,13 @@ int updc_ctx_destroy(updc_ctx *ctx) {
     });
 }
 
+int updc_ctx_set_flags(updc_ctx *ctx, unsigned flags) {
+    return guarded([&] {
+        need(ctx != nullptr, UPDC_ERR_ARG, "null context");
+        ctx->flags = flags;
+    });
+}
+
 int updc_ctx_set_preprocess_workers(updc_ctx *ctx, int workers) {
     return guarded([&] {
         need(ctx && workers >= 1, UPDC_ERR_ARG, "invalid worker count");
@@ -1151,6 +1162,17 @@ int updc_get_frozen(updc_ctx *ctx, int k, uint8_t *out) {
         NEED_PREPARED(ctx);
         need(out && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
         ctx->pipe->get_frozen(*ctx, k, out);
+    });
+}
+int updc_get_exposure_info(updc_ctx *ctx, int k, uint64_t *nin, uint64_t *nlinks, uint32_t *encoding,
+                           uint64_t *ncsc) {
+    return guarded([&] {
+        need(ctx && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        const HostExp &e = *ctx->ex[k];
+        if (nin) *nin = e.nin;
+        if (nlinks) *nlinks = e.nlinks;
+        if (encoding) *encoding = e.encoding;
+        if (ncsc) *ncsc = e.ncsc;
     });
 }
 int updc_reset_frozen(updc_ctx *ctx) {

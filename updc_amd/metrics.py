@@ -1,0 +1,53 @@
+"""Image comparison metrics and the algorithmic byte model of one iteration.
+
+stats() follows the reference's tools/compare_fits.py:10-29 (max_abs, rms,
+max_rel_masked with |a| > 1e-8 max|a|, sums, negative and non-finite counts)
+and adds the normalised L-infinity max|d| / max|a|.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def stats(ref: np.ndarray, new: np.ndarray) -> dict:
+    a = np.asarray(ref, np.float64)
+    b = np.asarray(new, np.float64)
+    d = b - a
+    scale = max(float(np.nanmax(np.abs(a))), 1e-30)
+    mask = np.abs(a) > 1e-8 * scale
+    rel = np.abs(d[mask] / a[mask]) if np.any(mask) else np.array([0.0])
+    return dict(max_abs=float(np.nanmax(np.abs(d))), rms=float(np.sqrt(np.nanmean(d * d))),
+                max_rel_masked=float(np.nanmax(rel)), ref_sum=float(np.nansum(a)), new_sum=float(np.nansum(b)),
+                ref_negative=int(np.sum(a < 0)), new_negative=int(np.sum(b < 0)),
+                nonfinite_ref=int(np.sum(~np.isfinite(a))), nonfinite_new=int(np.sum(~np.isfinite(b))),
+                norm_linf=float(np.nanmax(np.abs(d)) / scale),
+                rel_rms=float(np.sqrt(np.nanmean(d * d)) / max(np.sqrt(np.nanmean(a * a)), 1e-30)))
+
+
+def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[int], ncsc: list[int],
+                lm_bytes: list[int], e: int, c: int, e_d: int) -> dict:
+    """Algorithmic HBM bytes per launch of each iteration kernel: every
+    operand read once and every result written once (FFT-internal LDS passes
+    and re-reads excluded).  e/c = real/complex element bytes of the FFT
+    type, e_d = bytes of the denominator element."""
+    N = len(nin)
+    NC = FW // 2 + 1
+    nout = W * H
+    spec_x = H * NC * c
+    psf = N * FH * NC * c
+    spec_k = N * H * NC * c
+    pix = sum(nin)
+    lm = sum(lm_bytes)
+    csc = sum(4 * (nout + 1) + 8 * n for n in ncsc)
+    return {
+        "row_r2c_x": nout * e + spec_x,
+        "col_fwd_bcast": spec_x + psf + spec_k,
+        "row_c2r_blur": spec_k + N * nout * e,
+        # LinkMap + blur + sci + w (e) + frozen r/w (1+1) + wr (e)
+        "gather_freeze": lm + N * nout * e + pix * (3 * e + 2),
+        # CSC + wr + spectrum rows
+        "row_r2c_gather": csc + pix * e + spec_k,
+        "col_adj_accum": spec_k + psf + spec_x,
+        "row_c2r_corr": spec_x + nout * e,
+        "update": nout * (3 * e + e_d),
+    }

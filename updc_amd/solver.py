@@ -61,6 +61,7 @@ class Context:
         check(self.lib.updc_ctx_create(C.byref(h), self.W, self.H, int(fw), int(fh), _lib.PREC[precision], flags,
                                        int(device)))
         self.h = h
+        self.flags = flags
         self.nexp = 0
         self.nin: list[int] = []
         self.shapes: list[tuple[int, int]] = []
@@ -85,6 +86,10 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_graph(self, on: bool) -> None:
+        self.flags = (self.flags | FLAG_GRAPH) if on else (self.flags & ~FLAG_GRAPH)
+        check(self.lib.updc_ctx_set_flags(self.h, self.flags))
 
     # -- setup
     def add_exposure(self, e: Exposure) -> None:
@@ -181,6 +186,14 @@ class Context:
         out = np.empty(self.nin[k], np.uint8)
         check(self.lib.updc_get_frozen(self.h, k, out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
+
+    def exposure_info(self, k: int) -> dict:
+        nin, nl, ncsc = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        enc = C.c_uint32()
+        check(self.lib.updc_get_exposure_info(self.h, k, C.byref(nin), C.byref(nl), C.byref(enc), C.byref(ncsc)))
+        n = C.c_size_t()
+        check(self.lib.updc_linkmap_size(self.h, k, C.byref(n)))
+        return dict(nin=nin.value, nlinks=nl.value, encoding=enc.value, ncsc=ncsc.value, linkmap_bytes=n.value)
 
     def reset_frozen(self) -> None:
         check(self.lib.updc_reset_frozen(self.h))
