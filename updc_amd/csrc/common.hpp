@@ -130,9 +130,13 @@ struct FftLen {
     const void *tw;
 };
 
+// LDS slot of FFT element x (identity; a bank swizzle measured slower on
+// gfx950 for these transforms).
+__device__ __forceinline__ int sw(int x) { return x; }
+
 // One Stockham (decimation in time, self-sorting) radix-R stage over
 // `1 << lg` interleaved transforms held in LDS: element n of line c lives at
-// buf[(n << lg) + c].  Butterfly b handles line c = b & (lines-1), index
+// buf[sw((n << lg) + c)].  Butterfly b handles line c = b & (lines-1), index
 // i = b >> lg, reads the R values i + q L/R, applies the twiddles
 // exp(-+2 pi i q k / (Ns R)) with k = i mod Ns, and writes the outputs to
 // (i / Ns) Ns R + k + q Ns.  All reads finish before the barrier, so the stage
@@ -155,7 +159,7 @@ __device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int Ns, uns
         if (b < nb) {
             const int c = b & lm, i = b >> lg;
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[r][q] = buf[((i + q * LR) << lg) + c];
+            for (int q = 0; q < R; ++q) v[r][q] = buf[sw(((i + q * LR) << lg) + c)];
         }
     }
     __syncthreads();
@@ -178,7 +182,7 @@ __device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int Ns, uns
             bfly<R, INV>(v[r]);
             const int j = hi * Ns * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[((j + q * Ns) << lg) + c] = v[r][q];
+            for (int q = 0; q < R; ++q) buf[sw(((j + q * Ns) << lg) + c)] = v[r][q];
         }
     }
     __syncthreads();

@@ -21,12 +21,23 @@ template <class T> __device__ __forceinline__ T csc_gather(const ExpDev &e, cons
     // scatter[t] = sum over source pixels p (ascending) of fma(frac, w[p], .)
     // from 0 -- the same fmaf chain as updc_core.c:1173-1176 / :1064, since the
     // reference visits pixels in ascending p and each (p,t) pair once.
-    uint32_t a = e.toff[t];
-    const uint32_t end = e.toff[t + 1];
+    const uint32_t a = e.toff[t];
+    const uint32_t n = e.toff[t + 1] - a;
+    // entries and their w loads issued together, then the chain in order
+    constexpr int U = 6;
+    uint2 en[U];
+    T wv[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) en[j] = (uint32_t)j < n ? e.tent[a + j] : make_uint2(0u, 0u);
+#pragma unroll
+    for (int j = 0; j < U; ++j) wv[j] = (uint32_t)j < n ? w[en[j].x] : (T)0;
     T acc = (T)0;
-    for (; a < end; ++a) {
-        const uint2 en = e.tent[a];
-        acc = fmat((T)__uint_as_float(en.y), w[en.x], acc);
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+        if ((uint32_t)j < n) acc = fmat((T)__uint_as_float(en[j].y), wv[j], acc);
+    for (uint32_t j = U; j < n; ++j) {
+        const uint2 x = e.tent[a + j];
+        acc = fmat((T)__uint_as_float(x.y), w[x.x], acc);
     }
     return acc;
 }
@@ -45,7 +56,7 @@ __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restric
             const int c = 2 * n;
             const T x0 = c < win ? row[c] : (T)0;
             const T x1 = c + 1 < win ? row[c + 1] : (T)0;
-            z[n] = cx<T>(x0, x1);
+            z[sw(n)] = cx<T>(x0, x1);
         }
     } else {
         const ExpDev &e = ex[b];
@@ -55,7 +66,7 @@ __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restric
             const int c = 2 * n;
             const T x0 = c < W ? csc_gather<T>(e, w, t0 + c) : (T)0;
             const T x1 = c + 1 < W ? csc_gather<T>(e, w, t0 + c + 1) : (T)0;
-            z[n] = cx<T>(x0, x1);
+            z[sw(n)] = cx<T>(x0, x1);
         }
     }
     __syncthreads();
@@ -64,8 +75,8 @@ __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restric
     Cx<T> *o = out + (size_t)b * out_bstride + (size_t)r * out_pitch;
     const T half = (T)0.5;
     for (int v = threadIdx.x; v <= M; v += NT) {
-        const Cx<T> zv = z[v == M ? 0 : v];
-        const Cx<T> zm = cconj(z[v == 0 ? 0 : M - v]);
+        const Cx<T> zv = z[sw(v == M ? 0 : v)];
+        const Cx<T> zm = cconj(z[sw(v == 0 ? 0 : M - v)]);
         const Cx<T> fe = cscale(zv + zm, half);
         const Cx<T> d = zv - zm;
         const Cx<T> fo = cx<T>(d.im * half, -d.re * half);  // (zv - zm) / (2i)
@@ -87,14 +98,14 @@ __global__ __launch_bounds__(NT) void k_row_c2r(FftLen f, const Cx<T> *__restric
         const Cx<T> ym = cconj(y[M - v]);
         const Cx<T> e = yv + ym;
         const Cx<T> o = cmul(yv - ym, cconj(post[v]));
-        z[v] = cx<T>(e.re - o.im, e.im + o.re);
+        z[sw(v)] = cx<T>(e.re - o.im, e.im + o.re);
     }
     __syncthreads();
     fft_lds<true, T, NT, VMAX>(z, f, 0);
     T *orow = out + (size_t)b * out_bstride + (size_t)r * out_pitch;
     for (int n = threadIdx.x; n < M; n += NT) {
         const int c = 2 * n;
-        const Cx<T> v = z[n];
+        const Cx<T> v = z[sw(n)];
         if (c < wout) orow[c] = v.re * scale;
         if (c + 1 < wout) orow[c + 1] = v.im * scale;
     }
@@ -118,7 +129,7 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch,
     auto load = [&](const Cx<T> *src) {
         for (int e = tid; e < tot; e += NT) {
             const int n = e >> lg, col = c0 + (e & lm);
-            buf[e] = (n < hin && col < nc) ? src[(size_t)n * pitch + col] : cx<T>((T)0, (T)0);
+            buf[sw(e)] = (n < hin && col < nc) ? src[(size_t)n * pitch + col] : cx<T>((T)0, (T)0);
         }
         __syncthreads();
     };
@@ -126,7 +137,7 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch,
         const int lim = hout << lg;
         for (int e = tid; e < lim; e += NT) {
             const int n = e >> lg, col = c0 + (e & lm);
-            if (col < nc) dst[(size_t)n * pitch + col] = buf[e];
+            if (col < nc) dst[(size_t)n * pitch + col] = buf[sw(e)];
         }
     };
 
@@ -143,19 +154,17 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch,
 #pragma unroll
         for (int j = 0; j < XMAX; ++j) {
             const int e = tid + j * NT;
-            xr[j] = e < tot ? buf[e] : cx<T>((T)0, (T)0);
+            xr[j] = e < tot ? buf[sw(e)] : cx<T>((T)0, (T)0);
         }
         for (int k = 0; k < nexp; ++k) {
             __syncthreads();
-            const Cx<T> *pk = psf + (size_t)k * psf_bs;
+            const Cx<T> *pk = psf + (size_t)k * psf_bs + (size_t)tile * tot;  // tile-major, contiguous
 #pragma unroll
             for (int j = 0; j < XMAX; ++j) {
                 const int e = tid + j * NT;
                 if (e < tot) {
-                    const int n = e >> lg, col = c0 + (e & lm);
-                    const Cx<T> p = col < nc ? pk[(size_t)n * pitch + col] : cx<T>((T)0, (T)0);
                     // (a+ib)(c+id) as a*c-b*d, a*d+b*c (updc_core.c:756-758, adjoint=0)
-                    buf[e] = cmul(xr[j], p);
+                    buf[sw(e)] = cmul(xr[j], pk[e]);
                 }
             }
             __syncthreads();
@@ -170,14 +179,13 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch,
             __syncthreads();
             load(in + (size_t)k * in_bs);
             fft_lds<false, T, NT, VMAX>(buf, f, lg);
-            const Cx<T> *pk = psf + (size_t)k * psf_bs;
+            const Cx<T> *pk = psf + (size_t)k * psf_bs + (size_t)tile * tot;  // tile-major, contiguous
 #pragma unroll
             for (int j = 0; j < XMAX; ++j) {
                 const int e = tid + j * NT;
                 if (e < tot) {
-                    const int n = e >> lg, col = c0 + (e & lm);
-                    const Cx<T> p = col < nc ? pk[(size_t)n * pitch + col] : cx<T>((T)0, (T)0);
-                    const Cx<T> s = buf[e];
+                    const Cx<T> p = pk[e];
+                    const Cx<T> s = buf[sw(e)];
                     // re += a*c + b*d ; im += b*c - a*d  (updc_core.c:991-993)
                     const T pre = s.re * p.re + s.im * p.im;
                     const T pim = s.im * p.re - s.re * p.im;
@@ -190,11 +198,51 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch,
 #pragma unroll
         for (int j = 0; j < XMAX; ++j) {
             const int e = tid + j * NT;
-            if (e < tot) buf[e] = cx<T>((T)acc[j].re, (T)acc[j].im);
+            if (e < tot) buf[sw(e)] = cx<T>((T)acc[j].re, (T)acc[j].im);
         }
         __syncthreads();
         fft_lds<true, T, NT, VMAX>(buf, f, lg);
         store(out);
+    }
+}
+
+template <class T>
+__global__ void k_to_tiles(const Cx<T> *__restrict__ src, int pitch, int nc, int FH, int lg, size_t total,
+                           Cx<T> *__restrict__ dst, bool back) {
+    const size_t tot = (size_t)FH << lg;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const size_t tile = i / tot, e = i % tot;
+        const int n = (int)(e >> lg), col = (int)(tile << lg) + (int)(e & ((1u << lg) - 1));
+        if (!back) dst[i] = col < nc ? src[(size_t)n * pitch + col] : cx<T>((T)0, (T)0);
+        else if (col < nc) dst[(size_t)n * pitch + col] = src[i];
+    }
+}
+
+// Elementwise spectrum kernels of the per-exposure form (the reference's
+// ws_multiply_spectrum / accumulate_adjoint_spectrum, updc_core.c:753-760,
+// 988-995), over [FH][NCP] spectra with tile-major PSF spectra.
+template <class T, class TA, int OP>
+__global__ void k_spec(const Cx<T> *__restrict__ a, const Cx<T> *__restrict__ psf, Cx<T> *__restrict__ out,
+                       Cx<TA> *__restrict__ acc, int pitch, int nc, int FH, int lg) {
+    const size_t n = (size_t)FH * nc;
+    const size_t tot = (size_t)FH << lg;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int r = (int)(i / nc), col = (int)(i % nc);
+        const size_t g = (size_t)r * pitch + col;
+        if (OP == 2) {
+            out[g] = cx<T>((T)acc[g].re, (T)acc[g].im);
+            continue;
+        }
+        const size_t tix = (size_t)(col >> lg) * tot + ((size_t)r << lg) + (col & ((1 << lg) - 1));
+        const Cx<T> p = psf[tix], v = a[g];
+        if (OP == 0) {
+            out[g] = cmul(v, p);
+        } else {
+            const T pre = v.re * p.re + v.im * p.im;
+            const T pim = v.im * p.re - v.re * p.im;
+            acc[g].re += (TA)pre;
+            acc[g].im += (TA)pim;
+        }
     }
 }
 
@@ -207,6 +255,9 @@ template <class T> struct ColCfg;
 template <> struct ColCfg<float> { static constexpr int NT = 512, VMAX = 16, XMAX = 16; };
 template <> struct ColCfg<double> { static constexpr int NT = 512, VMAX = 16, XMAX = 12; };
 static constexpr size_t LDS_BUDGET = 150 * 1024;
+// LDS elements for n FFT values: sw() permutes inside 16-element blocks, so
+// buffers are rounded up to whole blocks.
+static inline size_t lds_elems(size_t n) { return (n + 15) & ~(size_t)15; }
 
 static bool factor_radices(int L, FftLen &f) {
     int n = L, k = 0;
@@ -269,13 +320,13 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1; s.NCP = (s.NC + 15) / 16 * 16;
     if (!factor_radices(s.M, row)) return "FFT width/2 is not 7-smooth";
     if (!factor_radices(FH, col)) return "FFT height is not 7-smooth";
-    if ((size_t)s.M * sizeof(Cx<T>) > LDS_BUDGET || !fits(row, 1, ROW_NT, ROW_VMAX))
+    if (lds_elems(s.M) * sizeof(Cx<T>) > LDS_BUDGET || !fits(row, 1, ROW_NT, ROW_VMAX))
         return "FFT width too large for the LDS-resident row pass";
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
     row_nt = ROW_NT; row_vmax = ROW_VMAX; col_nt = CNT; col_vmax = CVMAX;
     col_lines = 0;
     for (int t = 16; t >= 1; t >>= 1) {
-        if ((size_t)t * FH * sizeof(Cx<T>) > LDS_BUDGET) continue;
+        if (lds_elems((size_t)t * FH) * sizeof(Cx<T>) > LDS_BUDGET) continue;
         if ((long)t * FH > (long)CNT * CXMAX) continue;
         if (!fits(col, t, CNT, CVMAX)) continue;
         col_lines = t;
@@ -293,8 +344,8 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (!post) return err;
     row.tw = tw_row;
     col.tw = tw_col;
-    const int row_lds = (int)(s.M * sizeof(Cx<T>));
-    const int col_lds = (int)((size_t)col_lines * FH * sizeof(Cx<T>));
+    const int row_lds = (int)(lds_elems(s.M) * sizeof(Cx<T>));
+    const int col_lds = (int)(lds_elems((size_t)col_lines * FH) * sizeof(Cx<T>));
     void *fns_row[] = {row_r2c_fn<T, ROW_NT, ROW_VMAX, ROW_PLAIN>(), row_r2c_fn<T, ROW_NT, ROW_VMAX, ROW_GATHER>(),
                        row_c2r_fn<T, ROW_NT, ROW_VMAX>()};
     for (void *fn : fns_row)
@@ -309,6 +360,42 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     return "";
 }
 
+template <class T> hipError_t Fft2<T>::to_tiles(const Cx<T> *src, Cx<T> *dst, hipStream_t st) {
+    const size_t total = tile_stride();
+    hipLaunchKernelGGL((k_to_tiles<T>), dim3((unsigned)std::min<size_t>((total + 255) / 256, 16384)), dim3(256), 0, st,
+                       src, s.NCP, s.NC, s.FH, col_lg, total, dst, false);
+    return hipGetLastError();
+}
+template <class T> hipError_t Fft2<T>::from_tiles(const Cx<T> *src, Cx<T> *dst, hipStream_t st) {
+    const size_t total = tile_stride();
+    hipLaunchKernelGGL((k_to_tiles<T>), dim3((unsigned)std::min<size_t>((total + 255) / 256, 16384)), dim3(256), 0, st,
+                       src, s.NCP, s.NC, s.FH, col_lg, total, dst, true);
+    return hipGetLastError();
+}
+
+template <class T> hipError_t Fft2<T>::spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>(((size_t)s.FH * s.NC + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_spec<T, T, 0>), dim3(g), dim3(256), 0, st, a, psf_k, out, (Cx<T> *)nullptr, s.NCP, s.NC,
+                       s.FH, col_lg);
+    return hipGetLastError();
+}
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::spec_acc(const Cx<T> *a, const Cx<T> *psf_k, Cx<TA> *acc, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>(((size_t)s.FH * s.NC + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_spec<T, TA, 1>), dim3(g), dim3(256), 0, st, a, psf_k, (Cx<T> *)nullptr, acc, s.NCP, s.NC,
+                       s.FH, col_lg);
+    return hipGetLastError();
+}
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::spec_cast(const Cx<TA> *acc, Cx<T> *out, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>(((size_t)s.FH * s.NC + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_spec<T, TA, 2>), dim3(g), dim3(256), 0, st, (const Cx<T> *)nullptr, (const Cx<T> *)nullptr,
+                       out, (Cx<TA> *)acc, s.NCP, s.NC, s.FH, col_lg);
+    return hipGetLastError();
+}
+
 template <class T> void Fft2<T>::release() {
     if (tw_row) hipFree(tw_row);
     if (tw_col) hipFree(tw_col);
@@ -320,7 +407,7 @@ template <class T>
 hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pitch, int win, const ExpDev *ex,
                             const T *wr, int W, Cx<T> *out, size_t out_bstride, int rows, int batch, hipStream_t st) {
     if (rows <= 0 || batch <= 0) return hipSuccess;
-    const size_t lds = s.M * sizeof(Cx<T>);
+    const size_t lds = lds_elems(s.M) * sizeof(Cx<T>);
     dim3 grid(rows, batch);
     if (mode == ROW_PLAIN)
         hipLaunchKernelGGL((k_row_r2c<T, ROW_NT, ROW_VMAX, ROW_PLAIN>), grid, dim3(ROW_NT), lds, st, row, post, in,
@@ -335,7 +422,7 @@ template <class T>
 hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, T *out, size_t out_bstride, int out_pitch, int wout,
                             T scale, int rows, int batch, hipStream_t st) {
     if (rows <= 0 || batch <= 0) return hipSuccess;
-    const size_t lds = s.M * sizeof(Cx<T>);
+    const size_t lds = lds_elems(s.M) * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_row_c2r<T, ROW_NT, ROW_VMAX>), dim3(rows, batch), dim3(ROW_NT), lds, st, row, post, in,
                        in_bstride, s.NCP, out, out_bstride, out_pitch, wout, scale);
     return hipGetLastError();
@@ -348,7 +435,7 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, Cx<T> 
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
     const int ntiles = (s.NC + col_lines - 1) / col_lines;
     const int gx = (ntiles + 7) / 8 * 8;
-    const size_t lds = (size_t)col_lines * s.FH * sizeof(Cx<T>);
+    const size_t lds = lds_elems((size_t)col_lines * s.FH) * sizeof(Cx<T>);
     hin = std::min(hin, s.FH);
     hout = std::min(hout, s.FH);
     switch (mode) {
@@ -378,6 +465,12 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, Cx<T> 
 
 template struct Fft2<float>;
 template struct Fft2<double>;
+template hipError_t Fft2<float>::spec_acc<float>(const Cx<float> *, const Cx<float> *, Cx<float> *, hipStream_t);
+template hipError_t Fft2<float>::spec_acc<double>(const Cx<float> *, const Cx<float> *, Cx<double> *, hipStream_t);
+template hipError_t Fft2<double>::spec_acc<double>(const Cx<double> *, const Cx<double> *, Cx<double> *, hipStream_t);
+template hipError_t Fft2<float>::spec_cast<float>(const Cx<float> *, Cx<float> *, hipStream_t);
+template hipError_t Fft2<float>::spec_cast<double>(const Cx<double> *, Cx<float> *, hipStream_t);
+template hipError_t Fft2<double>::spec_cast<double>(const Cx<double> *, Cx<double> *, hipStream_t);
 template hipError_t Fft2<float>::colpass<float>(int, const Cx<float> *, size_t, Cx<float> *, size_t, const Cx<float> *,
                                             size_t, int, int, int, hipStream_t);
 template hipError_t Fft2<float>::colpass<double>(int, const Cx<float> *, size_t, Cx<float> *, size_t, const Cx<float> *,

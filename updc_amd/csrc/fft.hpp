@@ -46,6 +46,21 @@ template <class T> struct Fft2 {
     //   COL_FWD / COL_INV : in[b] -> out[b], b < batch
     //   COL_BCAST: X = FFT(in); out[k] = IFFT(X * psf[k]) for k < nexp
     //   COL_ADJ  : out = IFFT(sum_k conj(psf[k]) * FFT(in[k]))   (sum in TA)
+    //   (psf tile-major, psf_bstride = tile_stride())
+    // PSF spectra for COL_BCAST / COL_ADJ are stored tile-major: for column
+    // tile t (columns t*lines .. t*lines+lines-1), rows 0..FH-1, `lines`
+    // interleaved columns, so one workgroup reads its whole tile contiguously.
+    int ntiles() const { return (s.NC + col_lines - 1) / col_lines; }
+    size_t tile_stride() const { return (size_t)ntiles() * col_lines * s.FH; }  // elements per spectrum
+    hipError_t to_tiles(const Cx<T> *src, Cx<T> *dst, hipStream_t st);   // [FH][NCP] -> tile-major
+    hipError_t from_tiles(const Cx<T> *src, Cx<T> *dst, hipStream_t st); // tile-major -> [FH][NCP]
+    // Plain (per-exposure) form helpers over full FH x NCP spectra:
+    //   out = a * psf_k                         (psf tile-major)
+    //   acc += conj(psf_k) * a  (acc in TA)     (psf tile-major)
+    //   out = (T) acc
+    hipError_t spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st);
+    template <class TA> hipError_t spec_acc(const Cx<T> *a, const Cx<T> *psf_k, Cx<TA> *acc, hipStream_t st);
+    template <class TA> hipError_t spec_cast(const Cx<TA> *acc, Cx<T> *out, hipStream_t st);
     template <class TA>
     hipError_t colpass(int mode, const Cx<T> *in, size_t in_bstride, Cx<T> *out, size_t out_bstride, const Cx<T> *psf,
                    size_t psf_bstride, int hin, int hout, int batch, hipStream_t st);

@@ -254,7 +254,8 @@ template <class TF, class TA> struct Pipe : PipeBase {
     DevBuf<TF> sci, w, wr, blur, x, C, cov;
     DevBuf<TA> D;
     DevBuf<uint8_t> frozen;
-    DevBuf<Cx<TF>> psf_spec, spec_x, spec_k;
+    DevBuf<Cx<TF>> psf_spec, spec_x, spec_k, spec_tmp;
+    DevBuf<Cx<TA>> spec_acc;  // per-exposure form: summed adjoint spectrum
     DevBuf<ExpDiag> diag_cur, diag_hist;
     DevBuf<IterDiag> idiag_cur, idiag_hist;
     DevBuf<double> partials;
@@ -273,11 +274,14 @@ template <class TF, class TA> struct Pipe : PipeBase {
     int N(Ctx &c) const { return (int)c.ex.size(); }
     TF scale(Ctx &c) const { return (TF)1 / (TF)((size_t)c.FW * c.FH); }
 
-    // ---- PSF spectra P^_k for every exposure into `out` (N x FH x NCP).
+    // ---- PSF spectra P^_k for every exposure into `out`, tile-major
+    // (N x f.tile_stride(), the layout the column kernels read).
     template <class TD> static void psf_spectra(Ctx &c, Fft2<TD> &f, Cx<TD> *out) {
         const FftShape &s = f.s;
         DevBuf<TD> img;
         img.alloc((size_t)s.FW * s.FH);
+        DevBuf<Cx<TD>> spec;
+        spec.alloc((size_t)s.FH * s.NCP);
         DevBuf<float> raw;
         for (size_t k = 0; k < c.ex.size(); ++k) {
             HostExp &e = *c.ex[k];
@@ -296,9 +300,10 @@ template <class TF, class TA> struct Pipe : PipeBase {
             HIPCHK(hipMemcpyAsync(raw.p, e.psf.data(), cnt * sizeof(float), hipMemcpyHostToDevice, c.st));
             HIPCHK(fill<TD>(img.p, (TD)0, img.n, c.st));
             HIPCHK(psf_wrap<TD>(raw.p, e.pw, e.ph, e.psf_t, s.FW, s.FH, inv, img.p, c.st));
-            Cx<TD> *o = out + k * (size_t)s.FH * s.NCP;
+            Cx<TD> *o = spec.p;
             HIPCHK(f.row_r2c(ROW_PLAIN, img.p, 0, s.FW, s.FW, nullptr, nullptr, 0, o, 0, s.FH, 1, c.st));
             HIPCHK(f.template colpass<TD>(COL_FWD, o, 0, o, 0, nullptr, 0, s.FH, s.FH, 1, c.st));
+            HIPCHK(f.to_tiles(o, out + k * f.tile_stride(), c.st));
             HIPCHK(hipStreamSynchronize(c.st));
         }
     }
@@ -312,7 +317,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         sk.alloc((size_t)n * c.H * s.NCP);
         sx.alloc((size_t)c.H * s.NCP);
         HIPCHK(f.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, dex, wts, c.W, sk.p, (size_t)c.H * s.NCP, c.H, n, c.st));
-        HIPCHK(f.template colpass<TD>(COL_ADJ, sk.p, (size_t)c.H * s.NCP, sx.p, 0, pspec, (size_t)s.FH * s.NCP, c.H,
+        HIPCHK(f.template colpass<TD>(COL_ADJ, sk.p, (size_t)c.H * s.NCP, sx.p, 0, pspec, f.tile_stride(), c.H,
                                       c.H, n, c.st));
         const TD sc = (TD)1 / (TD)((size_t)s.FW * s.FH);
         HIPCHK(f.row_c2r(sx.p, 0, Dout, 0, c.W, c.W, sc, c.H, 1, c.st));
@@ -418,7 +423,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         upload_ex(c);
 
         // PSF spectra
-        psf_spec.alloc((size_t)n * s.FH * s.NCP);
+        psf_spec.alloc((size_t)n * fft.tile_stride());
         psf_spectra<TF>(c, fft, psf_spec.p);
 
         // initial drizzle (updc_core.c:905-983)
@@ -441,7 +446,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             if (!e2.empty()) throw Error(UPDC_ERR_ARG, e2);
             try {
                 DevBuf<Cx<double>> ps;
-                ps.alloc((size_t)n * f64.s.FH * f64.s.NCP);
+                ps.alloc((size_t)n * f64.tile_stride());
                 psf_spectra<double>(c, f64, ps.p);
                 DevBuf<double> wd;
                 wd.alloc(npix);
@@ -479,12 +484,54 @@ template <class TF, class TA> struct Pipe : PipeBase {
         ctr.alloc(1);
         HIPCHK(fill<unsigned int>(ticket.p, 0u, 1, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
-        stage_names = {"row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze",
-                       "row_r2c_gather", "col_adj_accum", "row_c2r_corr", "update"};
+    }
+
+    // Plain form (UPDC_FLAG_PER_EXPOSURE): the reference's dataflow, one launch
+    // per exposure and stage (updc_core.c:1140-1185, 1255-1287): X^ once;
+    // per exposure blur = crop C2R(X^ P^_k), detector, R2C(scatter),
+    // Â += conj(P^_k) F(scatter); one inverse of Â; update.
+    void launch_iteration_plain(Ctx &c, hipEvent_t *ev) {
+        const int n = N(c);
+        const FftShape &s = fft.s;
+        const size_t nout = c.nout();
+        const size_t kstride = (size_t)c.H * s.NCP;
+        const TF sc = scale(c);
+        auto mark = [&](int i) { if (ev) HIPCHK(hipEventRecord(ev[i], c.st)); };
+        if (!spec_tmp.p) {
+            spec_tmp.alloc((size_t)s.FH * s.NCP);
+            spec_acc.alloc((size_t)s.FH * s.NCP);
+        }
+        Cx<TF> *xh = spec_x.p;  // FH rows after the column pass
+        mark(0);
+        HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, xh, 0, c.H, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_FWD, xh, 0, xh, 0, nullptr, 0, c.H, s.FH, 1, c.st));
+        HIPCHK(fill<TA>((TA *)spec_acc.p, (TA)0, 2 * (size_t)s.FH * s.NCP, c.st));
+        mark(1);
+        for (int k = 0; k < n; ++k) {
+            const Cx<TF> *pk = psf_spec.p + (size_t)k * fft.tile_stride();
+            Cx<TF> *sk = spec_k.p + (size_t)k * kstride;
+            HIPCHK(fft.spec_mul(xh, pk, spec_tmp.p, c.st));
+            HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, sk, 0, nullptr, 0, s.FH, c.H, 1, c.st));
+            HIPCHK(fft.row_c2r(sk, 0, blur.p + (size_t)k * nout, 0, c.W, c.W, sc, c.H, 1, c.st));
+            HIPCHK((gather_freeze<TF, TA>(d_ex.p, k, k + 1, (uint32_t)c.ex[k]->nin, blur.p, nout, sci.p, w.p,
+                                          frozen.p, wr.p, diag_cur.p, idiag_cur.p, c.st)));
+            HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p + k, wr.p, c.W, sk, 0, c.H, 1, c.st));
+            HIPCHK(fft.template colpass<TA>(COL_FWD, sk, 0, spec_tmp.p, 0, nullptr, 0, c.H, s.FH, 1, c.st));
+            HIPCHK(fft.template spec_acc<TA>(spec_tmp.p, pk, spec_acc.p, c.st));
+        }
+        mark(2);
+        HIPCHK(fft.template spec_cast<TA>(spec_acc.p, spec_tmp.p, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, spec_x.p, 0, nullptr, 0, s.FH, c.H, 1, c.st));
+        HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
+        mark(3);
+        HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
+        mark(4);
+        HIPCHK(commit_diag(diag_cur.p, n, idiag_cur.p, diag_hist.p, idiag_hist.p, hist_cap, ctr.p, c.st));
     }
 
     // One iteration's launches (identical eager or under graph capture).
     void launch_iteration(Ctx &c, hipEvent_t *ev) {
+        if (c.flags & UPDC_FLAG_PER_EXPOSURE) return launch_iteration_plain(c, ev);
         const int n = N(c);
         const FftShape &s = fft.s;
         const size_t nout = c.nout();
@@ -494,7 +541,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         mark(0);
         HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
         mark(1);
-        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, (size_t)s.FH * s.NCP,
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, fft.tile_stride(),
                                         c.H, c.H, n, c.st));
         mark(2);
         HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, sc, c.H, n, c.st));
@@ -504,7 +551,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         mark(4);
         HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, n, c.st));
         mark(5);
-        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, (size_t)s.FH * s.NCP,
+        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, fft.tile_stride(),
                                         c.H, c.H, n, c.st));
         mark(6);
         HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
@@ -566,20 +613,26 @@ template <class TF, class TA> struct Pipe : PipeBase {
         }
         float total_ms = 0;
         HIPCHK(hipEventElapsedTime(&total_ms, it0, it1));
-        stage_ms.assign(9, 0.0);
-        stage_n.assign(9, 0);
+        const bool plain = (c.flags & UPDC_FLAG_PER_EXPOSURE) != 0;
+        const int nst = plain ? 4 : 8;
+        stage_names = plain ? std::vector<std::string>{"plain_xhat", "plain_exposures", "plain_adjoint_inverse",
+                                                       "update"}
+                            : std::vector<std::string>{"row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze",
+                                                       "row_r2c_gather", "col_adj_accum", "row_c2r_corr", "update"};
+        stage_ms.assign(nst + 1, 0.0);
+        stage_n.assign(nst + 1, 0);
         if (!use_graph) {
             for (int it = 0; it < niter; ++it)
-                for (int s = 0; s < 8; ++s) {
+                for (int s = 0; s < nst; ++s) {
                     float ms = 0;
                     HIPCHK(hipEventElapsedTime(&ms, ev[(size_t)it * nev + s], ev[(size_t)it * nev + s + 1]));
                     stage_ms[s] += ms;
                     stage_n[s] += 1;
                 }
-            for (int s = 0; s < 8; ++s) stage_ms[s] /= std::max(1, stage_n[s]);
+            for (int s = 0; s < nst; ++s) stage_ms[s] /= std::max(1, stage_n[s]);
         }
-        stage_ms[8] = total_ms / std::max(1, niter);
-        stage_n[8] = niter;
+        stage_ms[nst] = total_ms / std::max(1, niter);
+        stage_n[nst] = niter;
         for (auto &e : ev) (void)hipEventDestroy(e);
         (void)hipEventDestroy(it0);
         (void)hipEventDestroy(it1);
@@ -674,9 +727,13 @@ template <class TF, class TA> struct Pipe : PipeBase {
     }
     void get_psf_spectrum(Ctx &c, int k, double *out) override {
         const FftShape &s = fft.s;
+        DevBuf<Cx<TF>> tmp;
+        tmp.alloc((size_t)s.FH * s.NCP);
+        HIPCHK(fft.from_tiles(psf_spec.p + (size_t)k * fft.tile_stride(), tmp.p, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
         std::vector<double> row((size_t)s.NCP * 2);
         for (int r = 0; r < s.FH; ++r) {
-            to_host_f64((const TF *)(psf_spec.p + ((size_t)k * s.FH + r) * s.NCP), (size_t)s.NCP * 2, row.data(), c);
+            to_host_f64((const TF *)(tmp.p + (size_t)r * s.NCP), (size_t)s.NCP * 2, row.data(), c);
             std::memcpy(out + (size_t)r * s.NC * 2, row.data(), (size_t)s.NC * 2 * sizeof(double));
         }
     }
@@ -689,7 +746,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         from_host_f64(xin, nout, xs.p, c);
         const size_t kstride = (size_t)c.H * s.NCP;
         HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, (size_t)s.FH * s.NCP,
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, fft.tile_stride(),
                                         c.H, c.H, n, c.st));
         HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
@@ -704,7 +761,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         from_host_f64(xin, nout, xs.p, c);
         const size_t kstride = (size_t)c.H * s.NCP;
         HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, (size_t)s.FH * s.NCP,
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, fft.tile_stride(),
                                         c.H, c.H, n, c.st));
         HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
         HIPCHK(predict<TF>(d_ex.p, n, max_nin, blur.p, nout, wr.p, c.st));
@@ -741,7 +798,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const size_t kstride = (size_t)c.H * s.NCP;
         from_host_f64(v, c.npix(), wr.p, c);
         HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, n, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, (size_t)s.FH * s.NCP,
+        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, fft.tile_stride(),
                                         c.H, c.H, n, c.st));
         HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
         HIPCHK(hipStreamSynchronize(c.st));

@@ -458,70 +458,133 @@ hipError_t drizzle(const ExpDev *ex, int nexp, const TF *sci, const TF *w, int n
 // raw_mu; wr = w * ratio (updc_core.c:1152-1170).  F32: every step in f32 as
 // the reference.  MIXED: pred in f32, raw_mu, the test and the ratio in f64.
 // F64: everything in f64.
+// Grid-stride over chunks of 256 consecutive pixels of one exposure
+// (blockIdx.y).  A chunk owns one contiguous link range [off[p0], off[p0+256]),
+// staged in LDS with coalesced loads before each lane walks its own links
+// from LDS; chunks whose range exceeds the LDS capacity read their links from
+// HBM directly.  FreezeMu counters and the minimum raw_mu are reduced per
+// block and committed with one atomic per block (a per-wave atomic on the
+// exposure's counters serialises the whole launch).
+static constexpr int GATHER_NT = 256, GATHER_CAP = 4096, GATHER_BLOCKS = 1280, GATHER_UNROLL = 12;
+
 template <class TF, class TA>
-__global__ __launch_bounds__(256) void k_gather(const ExpDev *__restrict__ ex, int kbeg, const TF *__restrict__ blur,
-                                                size_t blur_bstride, const TF *__restrict__ sci,
-                                                const TF *__restrict__ w, uint8_t *__restrict__ frozen,
-                                                TF *__restrict__ wr, ExpDiag *diag, IterDiag *idiag) {
+__global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__ ex, int kbeg,
+                                                      const TF *__restrict__ blur, size_t blur_bstride,
+                                                      const TF *__restrict__ sci, const TF *__restrict__ w,
+                                                      uint8_t *__restrict__ frozen, TF *__restrict__ wr,
+                                                      ExpDiag *diag, IterDiag *idiag) {
+    __shared__ uint32_t s_tgt[GATHER_CAP];
+    __shared__ float s_frac[GATHER_CAP];
+    __shared__ unsigned int r_new[GATHER_NT / 64], r_tot[GATHER_NT / 64];
+    __shared__ TA r_min[GATHER_NT / 64];
     const int k = kbeg + blockIdx.y;
     const ExpDev &e = ex[k];
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    const bool d16 = e.encoding == UPDC_LM_DELTA16;
+    const TF *bl = blur + (size_t)k * blur_bstride;
     unsigned int nnew = 0, ntot = 0, bad = 0;
+    unsigned long long badidx = ~0ull;
     TA mn = (TA)INFINITY;
-    if (p < e.nin) {
-        const size_t gp = e.pix + p;
-        const TF wp = w[gp];
-        const uint32_t z0 = e.off[p], z1 = e.off[p + 1];
-        TF out = (TF)0;
-        if (wp > (TF)0 && z0 != z1) {
-            const TF *bl = blur + (size_t)k * blur_bstride;
-            TF pred = (TF)0;
-            if (e.encoding == UPDC_LM_DELTA16) {
-                const uint32_t b0 = e.base[p];
-                for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)e.frac[z], bl[b0 + (uint32_t)e.d16[z]], pred);
-            } else {
-                for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)e.frac[z], bl[e.t32[z]], pred);
-            }
-            TA raw;
-            if constexpr (sizeof(TA) == 4) raw = pred + e.mean_offset_f;
-            else raw = (TA)pred + (TA)e.mean_offset;
-            if (!isfinite(raw)) bad |= UPDC_E_MU;
-            mn = raw;
-            uint8_t fz = frozen[gp];
-            if (!fz && !(raw > (TA)0)) {
-                fz = 1;
-                frozen[gp] = 1;
-                nnew = 1;
-            }
-            TA ratio;
-            if (fz) {
-                ntot = 1;
-                ratio = (TA)1;
-            } else {
-                TA y;
-                if constexpr (sizeof(TA) == 4) y = sci[gp] + e.pedestal_f;
-                else y = (TA)sci[gp] + (TA)e.pedestal;
-                ratio = y / raw;
-                if (!isfinite(ratio)) bad |= UPDC_E_RATIO;
-            }
-            out = (TF)((TA)wp * ratio);
+    for (uint32_t p0 = blockIdx.x * GATHER_NT; p0 < e.nin; p0 += gridDim.x * GATHER_NT) {
+        const uint32_t p = p0 + threadIdx.x;
+        const uint32_t pend = min(p0 + (uint32_t)GATHER_NT, e.nin);
+        const uint32_t Z0 = e.off[p0], Z1 = e.off[pend];
+        const bool staged = Z1 - Z0 <= (uint32_t)GATHER_CAP;
+        __syncthreads();  // previous chunk's LDS reads are done
+        if (staged) {
+            // coalesced copies; DELTA16 keeps the raw delta, the lane adds its base
+            if (d16)
+                for (uint32_t z = Z0 + threadIdx.x; z < Z1; z += GATHER_NT) s_tgt[z - Z0] = (uint32_t)e.d16[z];
+            else
+                for (uint32_t z = Z0 + threadIdx.x; z < Z1; z += GATHER_NT) s_tgt[z - Z0] = e.t32[z];
+            for (uint32_t z = Z0 + threadIdx.x; z < Z1; z += GATHER_NT) s_frac[z - Z0] = e.frac[z];
         }
-        wr[gp] = out;
+        __syncthreads();
+        if (p < pend) {
+            const size_t gp = e.pix + p;
+            const TF wp = w[gp];
+            const uint32_t z0 = e.off[p], z1 = e.off[p + 1];
+            TF out = (TF)0;
+            if (wp > (TF)0 && z0 != z1) {
+                TF pred = (TF)0;
+                const uint32_t b0 = d16 ? e.base[p] : 0u;
+                if (staged) {
+                    // issue the first GATHER_UNROLL blur loads together, then run the
+                    // fmaf chain in link order (same operations, one memory round trip)
+                    const uint32_t nz = z1 - z0, l0 = z0 - Z0;
+                    TF vals[GATHER_UNROLL];
+#pragma unroll
+                    for (int j = 0; j < GATHER_UNROLL; ++j) vals[j] = (uint32_t)j < nz ? bl[b0 + s_tgt[l0 + j]] : (TF)0;
+#pragma unroll
+                    for (int j = 0; j < GATHER_UNROLL; ++j)
+                        if ((uint32_t)j < nz) pred = fmat((TF)s_frac[l0 + j], vals[j], pred);
+                    for (uint32_t j = GATHER_UNROLL; j < nz; ++j)
+                        pred = fmat((TF)s_frac[l0 + j], bl[b0 + s_tgt[l0 + j]], pred);
+                } else if (d16) {
+                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)e.frac[z], bl[b0 + (uint32_t)e.d16[z]], pred);
+                } else {
+                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)e.frac[z], bl[e.t32[z]], pred);
+                }
+                TA raw;
+                if constexpr (sizeof(TA) == 4) raw = pred + e.mean_offset_f;
+                else raw = (TA)pred + (TA)e.mean_offset;
+                unsigned int b = 0;
+                if (!isfinite(raw)) b |= UPDC_E_MU;
+                mn = raw < mn ? raw : mn;
+                uint8_t fz = frozen[gp];
+                if (!fz && !(raw > (TA)0)) {
+                    fz = 1;
+                    frozen[gp] = 1;
+                    ++nnew;
+                }
+                TA ratio;
+                if (fz) {
+                    ++ntot;
+                    ratio = (TA)1;
+                } else {
+                    TA y;
+                    if constexpr (sizeof(TA) == 4) y = sci[gp] + e.pedestal_f;
+                    else y = (TA)sci[gp] + (TA)e.pedestal;
+                    ratio = y / raw;
+                    if (!isfinite(ratio)) b |= UPDC_E_RATIO;
+                }
+                out = (TF)((TA)wp * ratio);
+                if (b) {
+                    bad |= b;
+                    badidx = badidx < p ? badidx : p;
+                }
+            }
+            wr[gp] = out;
+        }
     }
-    // wave reductions, then one atomic per wave
-    const unsigned long long bn = __ballot(nnew), bt = __ballot(ntot);
+    // block reduction, one atomic per block and quantity
     for (int o = 32; o >= 1; o >>= 1) {
+        nnew += __shfl_xor(nnew, o);
+        ntot += __shfl_xor(ntot, o);
         const TA other = __shfl_xor(mn, o);
         mn = other < mn ? other : mn;
     }
+    const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        if (bn) atomicAdd(&diag[k].n_new, (unsigned long long)__popcll(bn));
-        if (bt) atomicAdd(&diag[k].n_total, (unsigned long long)__popcll(bt));
-        if (mn < (TA)INFINITY) atomicMin(&diag[k].min_key, dkey((double)mn));
+        r_new[wv] = nnew;
+        r_tot[wv] = ntot;
+        r_min[wv] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int a = 0, t = 0;
+        TA m = (TA)INFINITY;
+        for (int i = 0; i < GATHER_NT / 64; ++i) {
+            a += r_new[i];
+            t += r_tot[i];
+            m = r_min[i] < m ? r_min[i] : m;
+        }
+        if (a) atomicAdd(&diag[k].n_new, (unsigned long long)a);
+        if (t) atomicAdd(&diag[k].n_total, (unsigned long long)t);
+        if (m < (TA)INFINITY) atomicMin(&diag[k].min_key, dkey((double)m));
     }
     if (bad) {
         atomicOr(&idiag->err, bad);
-        atomicMin(&idiag->err_index, (unsigned long long)p);
+        atomicMin(&idiag->err_index, badidx);
     }
 }
 template <class TF, class TA>
@@ -529,7 +592,10 @@ hipError_t gather_freeze(const ExpDev *ex, int kbeg, int kend, uint32_t max_nin,
                          size_t blur_bstride, const TF *sci, const TF *w, uint8_t *frozen, TF *wr, ExpDiag *diag,
                          IterDiag *idiag, hipStream_t st) {
     if (kend <= kbeg) return hipSuccess;
-    hipLaunchKernelGGL((k_gather<TF, TA>), dim3((max_nin + 255) / 256, kend - kbeg), dim3(256), 0, st, ex, kbeg, blur,
+    const int nk = kend - kbeg;
+    const unsigned chunks = (max_nin + GATHER_NT - 1) / GATHER_NT;
+    const unsigned gb = std::max(1u, std::min(chunks, (unsigned)std::max(1, GATHER_BLOCKS / nk)));
+    hipLaunchKernelGGL((k_gather<TF, TA>), dim3(gb, nk), dim3(GATHER_NT), 0, st, ex, kbeg, blur,
                        blur_bstride, sci, w, frozen, wr, diag, idiag);
     return hipGetLastError();
 }

@@ -87,6 +87,11 @@ class Context:
         except Exception:
             pass
 
+    def set_per_exposure(self, on: bool) -> None:
+        """Plain form: one launch per exposure and stage (no batching)."""
+        self.flags = (self.flags | FLAG_PER_EXPOSURE) if on else (self.flags & ~FLAG_PER_EXPOSURE)
+        check(self.lib.updc_ctx_set_flags(self.h, self.flags))
+
     def set_graph(self, on: bool) -> None:
         self.flags = (self.flags | FLAG_GRAPH) if on else (self.flags & ~FLAG_GRAPH)
         check(self.lib.updc_ctx_set_flags(self.h, self.flags))
