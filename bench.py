@@ -130,12 +130,22 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the FP32/batched timed run (for rocprofv3)")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="first run k_copy_ref over this many GiB (HBM counter scale reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the multi-GPU form (exposures sharded across GPUs, one all-reduce "
               "of the correction image per iteration) is a design, not a run; use --gpus 1", file=sys.stderr)
         return 2
     from updc_amd import metrics
+    if args.calibrate_copy > 0:
+        import ctypes as C
+        from updc_amd import _lib
+        nbytes = int(args.calibrate_copy * (1 << 30))
+        ms = C.c_double()
+        _lib.check(_lib.load().updc_copy_calibration(nbytes, 3, 0, C.byref(ms)))
+        log(f"k_copy_ref: {nbytes} B read + {nbytes} B written per launch in {ms.value:.4f} ms "
+            f"-> {2 * nbytes / (ms.value * 1e-3) / 1e9:.1f} GB/s")
     ds = make_dataset(args.config)
     K, Wm = args.steps, args.warmup
 

@@ -198,6 +198,13 @@ int updc_fft_c2r_f64(updc_ctx *ctx, const double *spectrum, double *image);
  * names[i] / ms[i] / launches[i] for i < *count (capacity in, count out). */
 int updc_get_stage_times(updc_ctx *ctx, char names[][32], double *ms, int *launches, int *count);
 
+/* Measurement aid with no counterpart in the reference: copies `bytes` bytes
+ * between two device buffers `reps` times with a streaming kernel
+ * (k_copy_ref) and returns the average time per copy.  It is the known-size
+ * workload against which HBM byte counters (FETCH_SIZE / WRITE_SIZE) are
+ * scaled. */
+int updc_copy_calibration(size_t bytes, int reps, int device, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,7 @@ SIGNATURES = {
     "updc_fft_r2c_f64": (C.c_int, [_P, _D, _D]),
     "updc_fft_c2r_f64": (C.c_int, [_P, _D, _D]),
     "updc_get_stage_times": (C.c_int, [_P, C.c_void_p, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "updc_copy_calibration": (C.c_int, [C.c_size_t, C.c_int, C.c_int, _D]),
 }
 
 _lib = None

@@ -117,7 +117,10 @@ template <int R, bool INV, class T> __device__ __forceinline__ void bfly(Cx<T> *
 // ---------------------------------------------------------------- FFT plan
 #define UPDC_MAX_STAGES 16
 // One transform length, split into Stockham radix stages.  tw points at the
-// device table tw[m] = exp(-2 pi i m / L), m in [0, L), of the FFT's real type.
+// device twiddle tables of the FFT's real type, one per stage: stage s (radix
+// R, Ns = product of the earlier radices) stores exp(-2 pi i q k / (Ns R)) at
+// tw[to[s] + k (R-1) + q - 1] for k < Ns, 1 <= q < R, so the R-1 twiddles of a
+// butterfly are adjacent and a wave's twiddle reads are contiguous.
 // For stage s, ns[s] is the product of the earlier radices and mg[s] =
 // ceil(2^32 / ns[s]), so i / ns[s] = umulhi(i, mg[s]) exactly for i, ns < 2^16
 // (ns > 1; the first stage, ns = 1, divides trivially).
@@ -127,39 +130,37 @@ struct FftLen {
     int rad[UPDC_MAX_STAGES];
     int ns[UPDC_MAX_STAGES];
     unsigned int mg[UPDC_MAX_STAGES];
+    int to[UPDC_MAX_STAGES];
     const void *tw;
 };
 
-// LDS slot of FFT element x (identity; a bank swizzle measured slower on
-// gfx950 for these transforms).
-__device__ __forceinline__ int sw(int x) { return x; }
-
 // One Stockham (decimation in time, self-sorting) radix-R stage over
 // `1 << lg` interleaved transforms held in LDS: element n of line c lives at
-// buf[sw((n << lg) + c)].  Butterfly b handles line c = b & (lines-1), index
+// buf[(n << lg) + c].  Butterfly b handles line c = b & (lines-1), index
 // i = b >> lg, reads the R values i + q L/R, applies the twiddles
 // exp(-+2 pi i q k / (Ns R)) with k = i mod Ns, and writes the outputs to
 // (i / Ns) Ns R + k + q Ns.  All reads finish before the barrier, so the stage
 // runs in place in one LDS buffer.  VMAX bounds the complex values one thread
 // holds per stage; the host checks R * ceil(nb / NT) <= VMAX.
 template <int R, bool INV, class T, int NT, int VMAX>
-__device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int Ns, unsigned int mg,
-                                          const Cx<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int cfix, int Ns, unsigned int mg,
+                                          const Cx<T> *__restrict__ tws) {
+    // cfix >= 0: transform only line cfix of the interleaved layout
     constexpr int NB = VMAX / R;
     static_assert(NB >= 1, "VMAX too small for radix");
+    const bool one = cfix >= 0;
     const int lm = (1 << lg) - 1;
-    const int nb = (L / R) << lg;
     const int LR = L / R;
-    const int tstride = L / (Ns * R);
+    const int nb = one ? LR : (LR << lg);
     Cx<T> v[NB][R];
     const int tid = threadIdx.x;
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
         const int b = tid + r * NT;
         if (b < nb) {
-            const int c = b & lm, i = b >> lg;
+            const int c = one ? cfix : (b & lm), i = one ? b : (b >> lg);
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[r][q] = buf[sw(((i + q * LR) << lg) + c)];
+            for (int q = 0; q < R; ++q) v[r][q] = buf[((i + q * LR) << lg) + c];
         }
     }
     __syncthreads();
@@ -167,14 +168,15 @@ __device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int Ns, uns
     for (int r = 0; r < NB; ++r) {
         const int b = tid + r * NT;
         if (b < nb) {
-            const int c = b & lm, i = b >> lg;
+            const int c = one ? cfix : (b & lm), i = one ? b : (b >> lg);
             // Ns == 1 (first stage): mg = 2^32 does not fit, and i / 1 = i
             const int hi = Ns == 1 ? i : (int)__umulhi((unsigned int)i, mg);
             const int k = i - hi * Ns;
             if (Ns > 1) {
+                const Cx<T> *wk = tws + k * (R - 1);
 #pragma unroll
                 for (int q = 1; q < R; ++q) {
-                    Cx<T> w = tw[q * k * tstride];
+                    Cx<T> w = wk[q - 1];
                     if (INV) w.im = -w.im;
                     v[r][q] = cmul(v[r][q], w);
                 }
@@ -182,26 +184,96 @@ __device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int Ns, uns
             bfly<R, INV>(v[r]);
             const int j = hi * Ns * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[sw(((j + q * Ns) << lg) + c)] = v[r][q];
+            for (int q = 0; q < R; ++q) buf[((j + q * Ns) << lg) + c] = v[r][q];
         }
     }
     __syncthreads();
 }
 
-// Full transform over the LDS buffer (natural order in, natural order out).
+// Stages [s0, s1) over the LDS buffer (single line, lg = 0).
 template <bool INV, class T, int NT, int VMAX>
-__device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg) {
-    const Cx<T> *tw = (const Cx<T> *)p.tw;
+__device__ __forceinline__ void fft_lds_range(Cx<T> *buf, const FftLen &p, int s0, int s1) {
+    for (int s = s0; s < s1; ++s) {
+        const int R = p.rad[s], Ns = p.ns[s];
+        const unsigned int mg = p.mg[s];
+        const Cx<T> *tw = (const Cx<T> *)p.tw + p.to[s];
+        switch (R) {
+            case 2: fft_stage<2, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
+            case 3: fft_stage<3, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
+            case 4: fft_stage<4, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
+            case 5: fft_stage<5, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
+            case 7: fft_stage<7, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
+            default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
+        }
+    }
+}
+
+// Register-resident edge stages of a transform whose first and last radix
+// are both RE (LR = L / RE).  Thread tid owns butterflies i_r = tid + r NT
+// (i_r < LR) and the RE values of each at element positions i_r + q LR, the
+// positions the first stage reads and the last stage writes.
+//   edge_first: stage 0 (Ns = 1, no twiddles) from registers into LDS
+//               (outputs at i RE + q), then a barrier.
+//   edge_last : the last stage (Ns = LR, k = i) from LDS back into registers,
+//               then a barrier so the buffer may be rewritten.
+template <int RE, bool INV, class T, int NT, int NBE>
+__device__ __forceinline__ void edge_first(Cx<T> (&rb)[NBE][RE], Cx<T> *buf, int LR) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < NBE; ++r) {
+        const int i = tid + r * NT;
+        if (i < LR) {
+            Cx<T> v[RE];
+#pragma unroll
+            for (int q = 0; q < RE; ++q) v[q] = rb[r][q];
+            bfly<RE, INV>(v);
+#pragma unroll
+            for (int q = 0; q < RE; ++q) buf[i * RE + q] = v[q];
+        }
+    }
+    __syncthreads();
+}
+template <int RE, bool INV, class T, int NT, int NBE>
+__device__ __forceinline__ void edge_last(Cx<T> (&rb)[NBE][RE], const Cx<T> *buf, int LR,
+                                          const Cx<T> *__restrict__ tws) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < NBE; ++r) {
+        const int i = tid + r * NT;
+        if (i < LR) {
+            Cx<T> v[RE];
+#pragma unroll
+            for (int q = 0; q < RE; ++q) v[q] = buf[i + q * LR];
+            const Cx<T> *wk = tws + i * (RE - 1);
+#pragma unroll
+            for (int q = 1; q < RE; ++q) {
+                Cx<T> w = wk[q - 1];
+                if (INV) w.im = -w.im;
+                v[q] = cmul(v[q], w);
+            }
+            bfly<RE, INV>(v);
+#pragma unroll
+            for (int q = 0; q < RE; ++q) rb[r][q] = v[q];
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform over the LDS buffer (natural order in, natural order out):
+// all `1 << lg` interleaved lines together (cfix < 0) or line cfix only.
+template <bool INV, class T, int NT, int VMAX>
+__device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg, int cfix = -1) {
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.rad[s], Ns = p.ns[s];
         const unsigned int mg = p.mg[s];
+        const Cx<T> *tw = (const Cx<T> *)p.tw + p.to[s];
         switch (R) {
-            case 2: fft_stage<2, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
-            case 3: fft_stage<3, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
-            case 4: fft_stage<4, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
-            case 5: fft_stage<5, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
-            case 7: fft_stage<7, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
-            default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, lg, Ns, mg, tw); break;
+            case 2: fft_stage<2, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
+            case 3: fft_stage<3, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
+            case 4: fft_stage<4, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
+            case 5: fft_stage<5, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
+            case 7: fft_stage<7, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
+            default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
         }
     }
 }

@@ -42,133 +42,157 @@ template <class T> __device__ __forceinline__ T csc_gather(const ExpDev &e, cons
     return acc;
 }
 
-template <class T, int NT, int VMAX, int MODE>
+// Row R2C: RB consecutive rows per workgroup (interleaved in LDS, row j at
+// slot (n << LG) + j), one half-length complex FFT per row, then the real
+// split written column-major, so RB rows of one column land contiguously.
+template <class T, int NT, int VMAX, int RB, int MODE>
 __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restrict__ post, const T *__restrict__ in,
                                                 size_t in_bstride, int in_pitch, int win,
                                                 const ExpDev *__restrict__ ex, const T *__restrict__ wr, int W,
-                                                Cx<T> *__restrict__ out, size_t out_bstride, int out_pitch) {
+                                                Cx<T> *__restrict__ out, size_t out_bstride, int RP, int rows) {
+    constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
+    static_assert((1 << LG) == RB, "RB must be 1, 2, 4 or 8");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int r = blockIdx.x, b = blockIdx.y, M = f.L;
-    if (MODE == ROW_PLAIN) {
-        const T *row = in + (size_t)b * in_bstride + (size_t)r * in_pitch;
-        for (int n = threadIdx.x; n < M; n += NT) {
-            const int c = 2 * n;
-            const T x0 = c < win ? row[c] : (T)0;
-            const T x1 = c + 1 < win ? row[c + 1] : (T)0;
-            z[sw(n)] = cx<T>(x0, x1);
-        }
-    } else {
-        const ExpDev &e = ex[b];
-        const T *w = wr + e.pix;
-        const uint32_t t0 = (uint32_t)r * (uint32_t)W;
-        for (int n = threadIdx.x; n < M; n += NT) {
-            const int c = 2 * n;
-            const T x0 = c < W ? csc_gather<T>(e, w, t0 + c) : (T)0;
-            const T x1 = c + 1 < W ? csc_gather<T>(e, w, t0 + c + 1) : (T)0;
-            z[sw(n)] = cx<T>(x0, x1);
+    const int r0 = blockIdx.x * RB, b = blockIdx.y, M = f.L;
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+        const int r = r0 + j;
+        if (MODE == ROW_PLAIN) {
+            const T *row = in + (size_t)b * in_bstride + (size_t)r * in_pitch;
+            for (int n = threadIdx.x; n < M; n += NT) {
+                const int c = 2 * n;
+                const T x0 = (r < rows && c < win) ? row[c] : (T)0;
+                const T x1 = (r < rows && c + 1 < win) ? row[c + 1] : (T)0;
+                z[(n << LG) + j] = cx<T>(x0, x1);
+            }
+        } else {
+            const ExpDev &e = ex[b];
+            const T *w = wr + e.pix;
+            const uint32_t t0 = (uint32_t)r * (uint32_t)W;
+            for (int n = threadIdx.x; n < M; n += NT) {
+                const int c = 2 * n;
+                const T x0 = (r < rows && c < W) ? csc_gather<T>(e, w, t0 + c) : (T)0;
+                const T x1 = (r < rows && c + 1 < W) ? csc_gather<T>(e, w, t0 + c + 1) : (T)0;
+                z[(n << LG) + j] = cx<T>(x0, x1);
+            }
         }
     }
     __syncthreads();
-    fft_lds<false, T, NT, VMAX>(z, f, 0);
+    fft_lds<false, T, NT, VMAX>(z, f, LG);  // the RB rows together
     // Split: X[v] = (Z[v] + Z*[M-v])/2 - i/2 exp(-2 pi i v/FW) (Z[v] - Z*[M-v]), v in [0, M].
-    Cx<T> *o = out + (size_t)b * out_bstride + (size_t)r * out_pitch;
+    Cx<T> *o = out + (size_t)b * out_bstride;
     const T half = (T)0.5;
-    for (int v = threadIdx.x; v <= M; v += NT) {
-        const Cx<T> zv = z[sw(v == M ? 0 : v)];
-        const Cx<T> zm = cconj(z[sw(v == 0 ? 0 : M - v)]);
+    for (int idx = threadIdx.x; idx < ((M + 1) << LG); idx += NT) {
+        const int v = idx >> LG, j = idx & (RB - 1);
+        if (r0 + j >= rows) continue;
+        const Cx<T> zv = z[((v == M ? 0 : v) << LG) + j];
+        const Cx<T> zm = cconj(z[((v == 0 ? 0 : M - v) << LG) + j]);
         const Cx<T> fe = cscale(zv + zm, half);
         const Cx<T> d = zv - zm;
         const Cx<T> fo = cx<T>(d.im * half, -d.re * half);  // (zv - zm) / (2i)
-        o[v] = fe + cmul(post[v], fo);
+        o[(size_t)v * RP + r0 + j] = fe + cmul(post[v], fo);
     }
 }
 
-template <class T, int NT, int VMAX>
+// Row C2R: RB rows per workgroup.  The RB-element column runs of the
+// spectrum (v = 0..M) are staged in LDS once, merged in place into
+// half-length complex rows (one thread forms both members of a (v, M-v) pair,
+// so the merge needs no second buffer), transformed together, and written
+// cropped and scaled, row-major.
+template <class T, int NT, int VMAX, int RB>
 __global__ __launch_bounds__(NT) void k_row_c2r(FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in,
-                                                size_t in_bstride, int in_pitch, T *__restrict__ out,
-                                                size_t out_bstride, int out_pitch, int wout, T scale) {
+                                                size_t in_bstride, int RP, T *__restrict__ out, size_t out_bstride,
+                                                int out_pitch, int wout, T scale, int rows) {
+    constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int r = blockIdx.x, b = blockIdx.y, M = f.L;
-    const Cx<T> *y = in + (size_t)b * in_bstride + (size_t)r * in_pitch;
-    // Merge: C[v] = E[v] + i O[v], E = Y[v] + Y*[M-v], O = (Y[v] - Y*[M-v]) exp(+2 pi i v/FW).
-    for (int v = threadIdx.x; v < M; v += NT) {
-        const Cx<T> yv = y[v];
-        const Cx<T> ym = cconj(y[M - v]);
-        const Cx<T> e = yv + ym;
-        const Cx<T> o = cmul(yv - ym, cconj(post[v]));
-        z[sw(v)] = cx<T>(e.re - o.im, e.im + o.re);
+    Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);  // (M + 1) * RB slots
+    const int r0 = blockIdx.x * RB, b = blockIdx.y, M = f.L;
+    const Cx<T> *y = in + (size_t)b * in_bstride;
+    for (int idx = threadIdx.x; idx < ((M + 1) << LG); idx += NT) {
+        const int v = idx >> LG, j = idx & (RB - 1);
+        z[idx] = r0 + j < rows ? y[(size_t)v * RP + r0 + j] : cx<T>((T)0, (T)0);
     }
     __syncthreads();
-    fft_lds<true, T, NT, VMAX>(z, f, 0);
-    T *orow = out + (size_t)b * out_bstride + (size_t)r * out_pitch;
-    for (int n = threadIdx.x; n < M; n += NT) {
-        const int c = 2 * n;
-        const Cx<T> v = z[sw(n)];
-        if (c < wout) orow[c] = v.re * scale;
-        if (c + 1 < wout) orow[c + 1] = v.im * scale;
+    // Merge: C[v] = E[v] + i O[v], E = Y[v] + Y*[M-v], O = (Y[v] - Y*[M-v]) exp(+2 pi i v/FW).
+    for (int idx = threadIdx.x; idx < ((M / 2 + 1) << LG); idx += NT) {
+        const int v = idx >> LG, j = idx & (RB - 1);
+        const int u = M - v;  // partner
+        const Cx<T> yv = z[(v << LG) + j], yu = z[(u << LG) + j];
+        const Cx<T> e1 = yv + cconj(yu), o1 = cmul(yv - cconj(yu), cconj(post[v]));
+        const Cx<T> cv = cx<T>(e1.re - o1.im, e1.im + o1.re);
+        if (v != 0 && u != v) {
+            const Cx<T> e2 = yu + cconj(yv), o2 = cmul(yu - cconj(yv), cconj(post[u]));
+            z[(u << LG) + j] = cx<T>(e2.re - o2.im, e2.im + o2.re);
+        }
+        z[(v << LG) + j] = cv;
+    }
+    __syncthreads();
+    fft_lds<true, T, NT, VMAX>(z, f, LG);  // the RB rows together
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+        if (r0 + j >= rows) continue;
+        T *orow = out + (size_t)b * out_bstride + (size_t)(r0 + j) * out_pitch;
+        for (int n = threadIdx.x; n < M; n += NT) {
+            const int c = 2 * n;
+            const Cx<T> v = z[(n << LG) + j];
+            if (c < wout) orow[c] = v.re * scale;
+            if (c + 1 < wout) orow[c + 1] = v.im * scale;
+        }
     }
 }
 
+// Column pass: one spectrum column per workgroup, read and written as one
+// contiguous column-major run.  BCAST keeps the X^ column in registers and
+// loops over all exposures; ADJ keeps the running sum over exposures in
+// registers; neither X^ nor the summed adjoint spectrum touches HBM.
 template <class T, class TA, int NT, int VMAX, int XMAX, int MODE>
-__global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch, int hin, int hout,
-                                            const Cx<T> *__restrict__ in, size_t in_bs, Cx<T> *__restrict__ out,
-                                            size_t out_bs, const Cx<T> *__restrict__ psf, size_t psf_bs, int nexp,
-                                            int ntiles) {
+__global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__restrict__ in, size_t in_bs, int in_rp,
+                                            Cx<T> *__restrict__ out, size_t out_bs, int out_rp,
+                                            const Cx<T> *__restrict__ psf, size_t psf_bs, int psf_rp, int hin,
+                                            int hout, int nexp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int tile = xcd_tile(blockIdx.x, ntiles);
-    if (tile < 0) return;
-    const int lines = 1 << lg, lm = lines - 1;
-    const int c0 = tile << lg;
+    const int col = xcd_tile(blockIdx.x, nc);
+    if (col < 0) return;
     const int FH = f.L;
-    const int tot = FH << lg;
     const int tid = threadIdx.x;
 
     auto load = [&](const Cx<T> *src) {
-        for (int e = tid; e < tot; e += NT) {
-            const int n = e >> lg, col = c0 + (e & lm);
-            buf[sw(e)] = (n < hin && col < nc) ? src[(size_t)n * pitch + col] : cx<T>((T)0, (T)0);
-        }
+        const Cx<T> *c = src + (size_t)col * in_rp;
+        for (int n = tid; n < FH; n += NT) buf[n] = n < hin ? c[n] : cx<T>((T)0, (T)0);
         __syncthreads();
     };
     auto store = [&](Cx<T> *dst) {
-        const int lim = hout << lg;
-        for (int e = tid; e < lim; e += NT) {
-            const int n = e >> lg, col = c0 + (e & lm);
-            if (col < nc) dst[(size_t)n * pitch + col] = buf[sw(e)];
-        }
+        Cx<T> *c = dst + (size_t)col * out_rp;
+        for (int n = tid; n < hout; n += NT) c[n] = buf[n];
     };
 
     if (MODE == COL_FWD || MODE == COL_INV) {
         const int b = blockIdx.y;
         load(in + (size_t)b * in_bs);
-        fft_lds<MODE == COL_INV, T, NT, VMAX>(buf, f, lg);
+        fft_lds<MODE == COL_INV, T, NT, VMAX>(buf, f, 0);
         store(out + (size_t)b * out_bs);
     } else if (MODE == COL_BCAST) {
-        // X^ column tile: forward FFT once, kept in registers across exposures.
         load(in);
-        fft_lds<false, T, NT, VMAX>(buf, f, lg);
+        fft_lds<false, T, NT, VMAX>(buf, f, 0);
         Cx<T> xr[XMAX];
 #pragma unroll
         for (int j = 0; j < XMAX; ++j) {
             const int e = tid + j * NT;
-            xr[j] = e < tot ? buf[sw(e)] : cx<T>((T)0, (T)0);
+            xr[j] = e < FH ? buf[e] : cx<T>((T)0, (T)0);
         }
         for (int k = 0; k < nexp; ++k) {
             __syncthreads();
-            const Cx<T> *pk = psf + (size_t)k * psf_bs + (size_t)tile * tot;  // tile-major, contiguous
+            const Cx<T> *pk = psf + (size_t)k * psf_bs + (size_t)col * psf_rp;
 #pragma unroll
             for (int j = 0; j < XMAX; ++j) {
                 const int e = tid + j * NT;
-                if (e < tot) {
-                    // (a+ib)(c+id) as a*c-b*d, a*d+b*c (updc_core.c:756-758, adjoint=0)
-                    buf[sw(e)] = cmul(xr[j], pk[e]);
-                }
+                // (a+ib)(c+id) as a*c-b*d, a*d+b*c (updc_core.c:756-758, adjoint=0)
+                if (e < FH) buf[e] = cmul(xr[j], pk[e]);
             }
             __syncthreads();
-            fft_lds<true, T, NT, VMAX>(buf, f, lg);
+            fft_lds<true, T, NT, VMAX>(buf, f, 0);
             store(out + (size_t)k * out_bs);
         }
     } else {  // COL_ADJ
@@ -178,14 +202,14 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch,
         for (int k = 0; k < nexp; ++k) {
             __syncthreads();
             load(in + (size_t)k * in_bs);
-            fft_lds<false, T, NT, VMAX>(buf, f, lg);
-            const Cx<T> *pk = psf + (size_t)k * psf_bs + (size_t)tile * tot;  // tile-major, contiguous
+            fft_lds<false, T, NT, VMAX>(buf, f, 0);
+            const Cx<T> *pk = psf + (size_t)k * psf_bs + (size_t)col * psf_rp;
 #pragma unroll
             for (int j = 0; j < XMAX; ++j) {
                 const int e = tid + j * NT;
-                if (e < tot) {
+                if (e < FH) {
                     const Cx<T> p = pk[e];
-                    const Cx<T> s = buf[sw(e)];
+                    const Cx<T> s = buf[e];
                     // re += a*c + b*d ; im += b*c - a*d  (updc_core.c:991-993)
                     const T pre = s.re * p.re + s.im * p.im;
                     const T pim = s.im * p.re - s.re * p.im;
@@ -198,66 +222,51 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, int lg, int pitch,
 #pragma unroll
         for (int j = 0; j < XMAX; ++j) {
             const int e = tid + j * NT;
-            if (e < tot) buf[sw(e)] = cx<T>((T)acc[j].re, (T)acc[j].im);
+            if (e < FH) buf[e] = cx<T>((T)acc[j].re, (T)acc[j].im);
         }
         __syncthreads();
-        fft_lds<true, T, NT, VMAX>(buf, f, lg);
+        fft_lds<true, T, NT, VMAX>(buf, f, 0);
         store(out);
-    }
-}
-
-template <class T>
-__global__ void k_to_tiles(const Cx<T> *__restrict__ src, int pitch, int nc, int FH, int lg, size_t total,
-                           Cx<T> *__restrict__ dst, bool back) {
-    const size_t tot = (size_t)FH << lg;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-        const size_t tile = i / tot, e = i % tot;
-        const int n = (int)(e >> lg), col = (int)(tile << lg) + (int)(e & ((1u << lg) - 1));
-        if (!back) dst[i] = col < nc ? src[(size_t)n * pitch + col] : cx<T>((T)0, (T)0);
-        else if (col < nc) dst[(size_t)n * pitch + col] = src[i];
     }
 }
 
 // Elementwise spectrum kernels of the per-exposure form (the reference's
 // ws_multiply_spectrum / accumulate_adjoint_spectrum, updc_core.c:753-760,
-// 988-995), over [FH][NCP] spectra with tile-major PSF spectra.
+// 988-995), over full column-major spectra of pitch RP.
 template <class T, class TA, int OP>
 __global__ void k_spec(const Cx<T> *__restrict__ a, const Cx<T> *__restrict__ psf, Cx<T> *__restrict__ out,
-                       Cx<TA> *__restrict__ acc, int pitch, int nc, int FH, int lg) {
-    const size_t n = (size_t)FH * nc;
-    const size_t tot = (size_t)FH << lg;
+                       Cx<TA> *__restrict__ acc, int RP, int nc, int FH) {
+    const size_t n = (size_t)nc * RP;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const int r = (int)(i / nc), col = (int)(i % nc);
-        const size_t g = (size_t)r * pitch + col;
+        if ((int)(i % RP) >= FH) continue;
         if (OP == 2) {
-            out[g] = cx<T>((T)acc[g].re, (T)acc[g].im);
+            out[i] = cx<T>((T)acc[i].re, (T)acc[i].im);
             continue;
         }
-        const size_t tix = (size_t)(col >> lg) * tot + ((size_t)r << lg) + (col & ((1 << lg) - 1));
-        const Cx<T> p = psf[tix], v = a[g];
+        const Cx<T> p = psf[i], v = a[i];
         if (OP == 0) {
-            out[g] = cmul(v, p);
+            out[i] = cmul(v, p);
         } else {
             const T pre = v.re * p.re + v.im * p.im;
             const T pim = v.im * p.re - v.re * p.im;
-            acc[g].re += (TA)pre;
-            acc[g].im += (TA)pim;
+            acc[i].re += (TA)pre;
+            acc[i].im += (TA)pim;
         }
     }
 }
 
 // ------------------------------------------------------------------ host side
-// Thread counts and per-thread register budgets.  One column workgroup owns
-// most of a CU's LDS, so it runs 8 waves (2 per SIMD, at most 256 VGPRs);
-// the budgets below keep the column kernels free of spills in both precisions.
-static constexpr int ROW_NT = 512, ROW_VMAX = 12;
+// Thread counts and per-thread register budgets.  A column workgroup holds
+// one FH-point column (44.8 KB f32 / 89.6 KB f64 at FH = 5600) and runs 8
+// waves; the budgets keep the column kernels free of spills.  Row
+// workgroups hold RB rows (RB x M points) to write RB-element column runs.
+template <class T> struct RowCfg;
+template <> struct RowCfg<float> { static constexpr int NT = 1024, VMAX = 16, RB = 4; };
+template <> struct RowCfg<double> { static constexpr int NT = 512, VMAX = 16, RB = 2; };
 template <class T> struct ColCfg;
 template <> struct ColCfg<float> { static constexpr int NT = 512, VMAX = 16, XMAX = 16; };
 template <> struct ColCfg<double> { static constexpr int NT = 512, VMAX = 16, XMAX = 12; };
 static constexpr size_t LDS_BUDGET = 150 * 1024;
-// LDS elements for n FFT values: sw() permutes inside 16-element blocks, so
-// buffers are rounded up to whole blocks.
-static inline size_t lds_elems(size_t n) { return (n + 15) & ~(size_t)15; }
 
 static bool factor_radices(int L, FftLen &f) {
     int n = L, k = 0;
@@ -279,15 +288,42 @@ static bool factor_radices(int L, FftLen &f) {
     return n == 1 && k <= UPDC_MAX_STAGES && L < 65536;
 }
 
-// Every stage must fit: R * ceil(lines*L/R / NT) <= VMAX.
+// `lines` interleaved transforms per stage must fit: R * ceil(lines*(L/R) / NT) <= VMAX.
 static bool fits(const FftLen &f, int lines, int nt, int vmax) {
     for (int s = 0; s < f.nst; ++s) {
         const int R = f.rad[s];
-        const long nb = (long)lines * (f.L / R);
-        const long per = (nb + nt - 1) / nt;
+        const long per = ((long)lines * (f.L / R) + nt - 1) / nt;
         if (per > vmax / R) return false;
     }
     return true;
+}
+
+// Per-stage twiddle tables (layout in common.hpp's FftLen), long double
+// arithmetic rounded once.
+template <class T> static Cx<T> *upload_stage_twiddles(FftLen &f, std::string &err) {
+    std::vector<Cx<T>> h;
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int s = 0; s < f.nst; ++s) {
+        const int R = f.rad[s], Ns = f.ns[s];
+        f.to[s] = (int)h.size();
+        for (int k = 0; k < Ns; ++k)
+            for (int q = 1; q < R; ++q) {
+                const long double a = twopi * (long double)(q * k) / (long double)(Ns * R);
+                Cx<T> w;
+                w.re = (T)cosl(a);
+                w.im = (T)(-sinl(a));
+                h.push_back(w);
+            }
+    }
+    if (h.empty()) h.push_back(Cx<T>{(T)1, (T)0});
+    Cx<T> *d = nullptr;
+    if (hipMalloc(&d, sizeof(Cx<T>) * h.size()) != hipSuccess) { err = "hipMalloc twiddles failed"; return nullptr; }
+    if (hipMemcpy(d, h.data(), sizeof(Cx<T>) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        err = "twiddle upload failed";
+        (void)hipFree(d);
+        return nullptr;
+    }
+    return d;
 }
 
 template <class T> static Cx<T> *upload_roots(int L, int count, int sign, std::string &err) {
@@ -303,163 +339,147 @@ template <class T> static Cx<T> *upload_roots(int L, int count, int sign, std::s
     if (hipMalloc(&d, sizeof(Cx<T>) * count) != hipSuccess) { err = "hipMalloc twiddles failed"; return nullptr; }
     if (hipMemcpy(d, h.data(), sizeof(Cx<T>) * count, hipMemcpyHostToDevice) != hipSuccess) {
         err = "twiddle upload failed";
-        hipFree(d);
+        (void)hipFree(d);
         return nullptr;
     }
     return d;
 }
 
-template <class T, int NT, int VMAX, int MODE> static void *row_r2c_fn() { return (void *)k_row_r2c<T, NT, VMAX, MODE>; }
-template <class T, int NT, int VMAX> static void *row_c2r_fn() { return (void *)k_row_c2r<T, NT, VMAX>; }
+template <class T, int MODE> static void *row_r2c_fn() {
+    return (void *)k_row_r2c<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, MODE>;
+}
+template <class T> static void *row_c2r_fn() {
+    return (void *)k_row_c2r<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB>;
+}
+// LDS of a row workgroup: RB interleaved rows of M points, plus the Nyquist
+// column the C2R merge stages.
+template <class T> static size_t row_lds(int M) { return (size_t)RowCfg<T>::RB * (M + 1) * sizeof(Cx<T>); }
 template <class T, class TA, int MODE> static void *col_fn() {
     return (void *)k_col<T, TA, ColCfg<T>::NT, ColCfg<T>::VMAX, ColCfg<T>::XMAX, MODE>;
 }
 
 template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (FW < 2 || FH < 1 || (FW & 1)) return "FFT width must be even and >= 2";
-    s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1; s.NCP = (s.NC + 15) / 16 * 16;
+    s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1;
     if (!factor_radices(s.M, row)) return "FFT width/2 is not 7-smooth";
     if (!factor_radices(FH, col)) return "FFT height is not 7-smooth";
-    if (lds_elems(s.M) * sizeof(Cx<T>) > LDS_BUDGET || !fits(row, 1, ROW_NT, ROW_VMAX))
-        return "FFT width too large for the LDS-resident row pass";
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
-    row_nt = ROW_NT; row_vmax = ROW_VMAX; col_nt = CNT; col_vmax = CVMAX;
-    col_lines = 0;
-    for (int t = 16; t >= 1; t >>= 1) {
-        if (lds_elems((size_t)t * FH) * sizeof(Cx<T>) > LDS_BUDGET) continue;
-        if ((long)t * FH > (long)CNT * CXMAX) continue;
-        if (!fits(col, t, CNT, CVMAX)) continue;
-        col_lines = t;
-        col_lg = 0;
-        while ((1 << col_lg) < t) ++col_lg;
-        break;
-    }
-    if (!col_lines) return "FFT height too large for the LDS-resident column pass";
+    if (row_lds<T>(s.M) > LDS_BUDGET || !fits(row, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX))
+        return "FFT width too large for the LDS-resident row pass";
+    if ((size_t)FH * sizeof(Cx<T>) > LDS_BUDGET || FH > CNT * CXMAX || !fits(col, 1, CNT, CVMAX))
+        return "FFT height too large for the LDS-resident column pass";
     std::string err;
-    tw_row = upload_roots<T>(s.M, s.M, -1, err);
+    tw_row = upload_stage_twiddles<T>(row, err);
     if (!tw_row) return err;
-    tw_col = upload_roots<T>(FH, FH, -1, err);
+    tw_col = upload_stage_twiddles<T>(col, err);
     if (!tw_col) return err;
     post = upload_roots<T>(FW, s.M + 1, -1, err);
     if (!post) return err;
     row.tw = tw_row;
     col.tw = tw_col;
-    const int row_lds = (int)(lds_elems(s.M) * sizeof(Cx<T>));
-    const int col_lds = (int)(lds_elems((size_t)col_lines * FH) * sizeof(Cx<T>));
-    void *fns_row[] = {row_r2c_fn<T, ROW_NT, ROW_VMAX, ROW_PLAIN>(), row_r2c_fn<T, ROW_NT, ROW_VMAX, ROW_GATHER>(),
-                       row_c2r_fn<T, ROW_NT, ROW_VMAX>()};
+    const int rlds = (int)row_lds<T>(s.M);
+    const int col_lds = (int)((size_t)FH * sizeof(Cx<T>));
+    void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>()};
     for (void *fn : fns_row)
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, row_lds) != hipSuccess)
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, rlds) != hipSuccess)
             return "cannot raise LDS limit for the row pass";
     void *fns_col[] = {col_fn<T, T, COL_FWD>(), col_fn<T, T, COL_INV>(), col_fn<T, T, COL_BCAST>(),
                        col_fn<T, float, COL_ADJ>(), col_fn<T, double, COL_ADJ>()};
     for (void *fn : fns_col)
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
             return "cannot raise LDS limit for the column pass";
-    lds_limit = col_lds;
     return "";
 }
 
-template <class T> hipError_t Fft2<T>::to_tiles(const Cx<T> *src, Cx<T> *dst, hipStream_t st) {
-    const size_t total = tile_stride();
-    hipLaunchKernelGGL((k_to_tiles<T>), dim3((unsigned)std::min<size_t>((total + 255) / 256, 16384)), dim3(256), 0, st,
-                       src, s.NCP, s.NC, s.FH, col_lg, total, dst, false);
-    return hipGetLastError();
-}
-template <class T> hipError_t Fft2<T>::from_tiles(const Cx<T> *src, Cx<T> *dst, hipStream_t st) {
-    const size_t total = tile_stride();
-    hipLaunchKernelGGL((k_to_tiles<T>), dim3((unsigned)std::min<size_t>((total + 255) / 256, 16384)), dim3(256), 0, st,
-                       src, s.NCP, s.NC, s.FH, col_lg, total, dst, true);
-    return hipGetLastError();
-}
-
-template <class T> hipError_t Fft2<T>::spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st) {
-    const unsigned g = (unsigned)std::min<size_t>(((size_t)s.FH * s.NC + 255) / 256, 16384);
-    hipLaunchKernelGGL((k_spec<T, T, 0>), dim3(g), dim3(256), 0, st, a, psf_k, out, (Cx<T> *)nullptr, s.NCP, s.NC,
-                       s.FH, col_lg);
-    return hipGetLastError();
-}
-template <class T>
-template <class TA>
-hipError_t Fft2<T>::spec_acc(const Cx<T> *a, const Cx<T> *psf_k, Cx<TA> *acc, hipStream_t st) {
-    const unsigned g = (unsigned)std::min<size_t>(((size_t)s.FH * s.NC + 255) / 256, 16384);
-    hipLaunchKernelGGL((k_spec<T, TA, 1>), dim3(g), dim3(256), 0, st, a, psf_k, (Cx<T> *)nullptr, acc, s.NCP, s.NC,
-                       s.FH, col_lg);
-    return hipGetLastError();
-}
-template <class T>
-template <class TA>
-hipError_t Fft2<T>::spec_cast(const Cx<TA> *acc, Cx<T> *out, hipStream_t st) {
-    const unsigned g = (unsigned)std::min<size_t>(((size_t)s.FH * s.NC + 255) / 256, 16384);
-    hipLaunchKernelGGL((k_spec<T, TA, 2>), dim3(g), dim3(256), 0, st, (const Cx<T> *)nullptr, (const Cx<T> *)nullptr,
-                       out, (Cx<TA> *)acc, s.NCP, s.NC, s.FH, col_lg);
-    return hipGetLastError();
-}
-
 template <class T> void Fft2<T>::release() {
-    if (tw_row) hipFree(tw_row);
-    if (tw_col) hipFree(tw_col);
-    if (post) hipFree(post);
+    if (tw_row) (void)hipFree(tw_row);
+    if (tw_col) (void)hipFree(tw_col);
+    if (post) (void)hipFree(post);
     tw_row = tw_col = post = nullptr;
 }
 
 template <class T>
 hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pitch, int win, const ExpDev *ex,
-                            const T *wr, int W, Cx<T> *out, size_t out_bstride, int rows, int batch, hipStream_t st) {
+                            const T *wr, int W, Cx<T> *out, size_t out_bstride, int rp_rows, int rows, int batch,
+                            hipStream_t st) {
     if (rows <= 0 || batch <= 0) return hipSuccess;
-    const size_t lds = lds_elems(s.M) * sizeof(Cx<T>);
-    dim3 grid(rows, batch);
+    constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
+    const size_t lds = row_lds<T>(s.M);
+    dim3 grid((rows + RB - 1) / RB, batch);
     if (mode == ROW_PLAIN)
-        hipLaunchKernelGGL((k_row_r2c<T, ROW_NT, ROW_VMAX, ROW_PLAIN>), grid, dim3(ROW_NT), lds, st, row, post, in,
-                           in_bstride, in_pitch, win, ex, wr, W, out, out_bstride, s.NCP);
+        hipLaunchKernelGGL((k_row_r2c<T, NT, VMAX, RB, ROW_PLAIN>), grid, dim3(NT), lds, st, row, post, in, in_bstride,
+                           in_pitch, win, ex, wr, W, out, out_bstride, rp(rp_rows), rows);
     else
-        hipLaunchKernelGGL((k_row_r2c<T, ROW_NT, ROW_VMAX, ROW_GATHER>), grid, dim3(ROW_NT), lds, st, row, post, in,
-                           in_bstride, in_pitch, win, ex, wr, W, out, out_bstride, s.NCP);
+        hipLaunchKernelGGL((k_row_r2c<T, NT, VMAX, RB, ROW_GATHER>), grid, dim3(NT), lds, st, row, post, in, in_bstride,
+                           in_pitch, win, ex, wr, W, out, out_bstride, rp(rp_rows), rows);
     return hipGetLastError();
 }
 
 template <class T>
-hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, T *out, size_t out_bstride, int out_pitch, int wout,
-                            T scale, int rows, int batch, hipStream_t st) {
+hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *out, size_t out_bstride,
+                            int out_pitch, int wout, T scale, int rows, int batch, hipStream_t st) {
     if (rows <= 0 || batch <= 0) return hipSuccess;
-    const size_t lds = lds_elems(s.M) * sizeof(Cx<T>);
-    hipLaunchKernelGGL((k_row_c2r<T, ROW_NT, ROW_VMAX>), dim3(rows, batch), dim3(ROW_NT), lds, st, row, post, in,
-                       in_bstride, s.NCP, out, out_bstride, out_pitch, wout, scale);
+    constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
+    const size_t lds = row_lds<T>(s.M);
+    hipLaunchKernelGGL((k_row_c2r<T, NT, VMAX, RB>), dim3((rows + RB - 1) / RB, batch), dim3(NT), lds, st, row, post,
+                       in, in_bstride, rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows);
     return hipGetLastError();
 }
 
 template <class T>
 template <class TA>
-hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, Cx<T> *out, size_t out_bstride, const Cx<T> *psf,
-                        size_t psf_bstride, int hin, int hout, int batch, hipStream_t st) {
+hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
+                            size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
+                            int hout, int batch, hipStream_t st) {
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
-    const int ntiles = (s.NC + col_lines - 1) / col_lines;
-    const int gx = (ntiles + 7) / 8 * 8;
-    const size_t lds = lds_elems((size_t)col_lines * s.FH) * sizeof(Cx<T>);
+    const int gx = (s.NC + 7) / 8 * 8;
+    const size_t lds = (size_t)s.FH * sizeof(Cx<T>);
     hin = std::min(hin, s.FH);
     hout = std::min(hout, s.FH);
+    const int irp = rp(in_rows), orp = rp(out_rows), prp = rp(s.FH);
     switch (mode) {
         case COL_FWD:
-            hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_FWD>), dim3(gx, batch), dim3(CNT), lds, st,
-                               col, s.NC, col_lg, s.NCP, hin, hout, in, in_bstride, out, out_bstride, psf,
-                               psf_bstride, batch, ntiles);
+            hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_FWD>), dim3(gx, batch), dim3(CNT), lds, st, col,
+                               s.NC, in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout,
+                               batch);
             break;
         case COL_INV:
-            hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_INV>), dim3(gx, batch), dim3(CNT), lds, st,
-                               col, s.NC, col_lg, s.NCP, hin, hout, in, in_bstride, out, out_bstride, psf,
-                               psf_bstride, batch, ntiles);
+            hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_INV>), dim3(gx, batch), dim3(CNT), lds, st, col,
+                               s.NC, in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout,
+                               batch);
             break;
         case COL_BCAST:
-            hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_BCAST>), dim3(gx, 1), dim3(CNT), lds, st,
-                               col, s.NC, col_lg, s.NCP, hin, hout, in, in_bstride, out, out_bstride, psf,
-                               psf_bstride, batch, ntiles);
+            hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_BCAST>), dim3(gx, 1), dim3(CNT), lds, st, col, s.NC,
+                               in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch);
             break;
         default:
-            hipLaunchKernelGGL((k_col<T, TA, CNT, CVMAX, CXMAX, COL_ADJ>), dim3(gx, 1), dim3(CNT), lds, st,
-                               col, s.NC, col_lg, s.NCP, hin, hout, in, in_bstride, out, out_bstride, psf,
-                               psf_bstride, batch, ntiles);
+            hipLaunchKernelGGL((k_col<T, TA, CNT, CVMAX, CXMAX, COL_ADJ>), dim3(gx, 1), dim3(CNT), lds, st, col, s.NC,
+                               in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch);
             break;
     }
+    return hipGetLastError();
+}
+
+template <class T> hipError_t Fft2<T>::spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>((spec_elems(s.FH) + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_spec<T, T, 0>), dim3(g), dim3(256), 0, st, a, psf_k, out, (Cx<T> *)nullptr, rp(s.FH), s.NC,
+                       s.FH);
+    return hipGetLastError();
+}
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::spec_acc(const Cx<T> *a, const Cx<T> *psf_k, Cx<TA> *acc, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>((spec_elems(s.FH) + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_spec<T, TA, 1>), dim3(g), dim3(256), 0, st, a, psf_k, (Cx<T> *)nullptr, acc, rp(s.FH), s.NC,
+                       s.FH);
+    return hipGetLastError();
+}
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::spec_cast(const Cx<TA> *acc, Cx<T> *out, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<size_t>((spec_elems(s.FH) + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_spec<T, TA, 2>), dim3(g), dim3(256), 0, st, (const Cx<T> *)nullptr, (const Cx<T> *)nullptr,
+                       out, (Cx<TA> *)acc, rp(s.FH), s.NC, s.FH);
     return hipGetLastError();
 }
 
@@ -471,13 +491,11 @@ template hipError_t Fft2<double>::spec_acc<double>(const Cx<double> *, const Cx<
 template hipError_t Fft2<float>::spec_cast<float>(const Cx<float> *, Cx<float> *, hipStream_t);
 template hipError_t Fft2<float>::spec_cast<double>(const Cx<double> *, Cx<float> *, hipStream_t);
 template hipError_t Fft2<double>::spec_cast<double>(const Cx<double> *, Cx<double> *, hipStream_t);
-template hipError_t Fft2<float>::colpass<float>(int, const Cx<float> *, size_t, Cx<float> *, size_t, const Cx<float> *,
-                                            size_t, int, int, int, hipStream_t);
-template hipError_t Fft2<float>::colpass<double>(int, const Cx<float> *, size_t, Cx<float> *, size_t, const Cx<float> *,
-                                             size_t, int, int, int, hipStream_t);
-template hipError_t Fft2<double>::colpass<double>(int, const Cx<double> *, size_t, Cx<double> *, size_t,
-                                              const Cx<double> *, size_t, int, int, int, hipStream_t);
-template hipError_t Fft2<double>::colpass<float>(int, const Cx<double> *, size_t, Cx<double> *, size_t,
-                                             const Cx<double> *, size_t, int, int, int, hipStream_t);
+template hipError_t Fft2<float>::colpass<float>(int, const Cx<float> *, size_t, int, Cx<float> *, size_t, int,
+                                                const Cx<float> *, size_t, int, int, int, hipStream_t);
+template hipError_t Fft2<float>::colpass<double>(int, const Cx<float> *, size_t, int, Cx<float> *, size_t, int,
+                                                 const Cx<float> *, size_t, int, int, int, hipStream_t);
+template hipError_t Fft2<double>::colpass<double>(int, const Cx<double> *, size_t, int, Cx<double> *, size_t, int,
+                                                  const Cx<double> *, size_t, int, int, int, hipStream_t);
 
 }  // namespace updc

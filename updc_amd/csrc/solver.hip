@@ -274,14 +274,12 @@ template <class TF, class TA> struct Pipe : PipeBase {
     int N(Ctx &c) const { return (int)c.ex.size(); }
     TF scale(Ctx &c) const { return (TF)1 / (TF)((size_t)c.FW * c.FH); }
 
-    // ---- PSF spectra P^_k for every exposure into `out`, tile-major
-    // (N x f.tile_stride(), the layout the column kernels read).
+    // ---- PSF spectra P^_k for every exposure into `out` (column-major,
+    // N x f.spec_elems(FH), pitch rp(FH)).
     template <class TD> static void psf_spectra(Ctx &c, Fft2<TD> &f, Cx<TD> *out) {
         const FftShape &s = f.s;
         DevBuf<TD> img;
         img.alloc((size_t)s.FW * s.FH);
-        DevBuf<Cx<TD>> spec;
-        spec.alloc((size_t)s.FH * s.NCP);
         DevBuf<float> raw;
         for (size_t k = 0; k < c.ex.size(); ++k) {
             HostExp &e = *c.ex[k];
@@ -300,10 +298,9 @@ template <class TF, class TA> struct Pipe : PipeBase {
             HIPCHK(hipMemcpyAsync(raw.p, e.psf.data(), cnt * sizeof(float), hipMemcpyHostToDevice, c.st));
             HIPCHK(fill<TD>(img.p, (TD)0, img.n, c.st));
             HIPCHK(psf_wrap<TD>(raw.p, e.pw, e.ph, e.psf_t, s.FW, s.FH, inv, img.p, c.st));
-            Cx<TD> *o = spec.p;
-            HIPCHK(f.row_r2c(ROW_PLAIN, img.p, 0, s.FW, s.FW, nullptr, nullptr, 0, o, 0, s.FH, 1, c.st));
-            HIPCHK(f.template colpass<TD>(COL_FWD, o, 0, o, 0, nullptr, 0, s.FH, s.FH, 1, c.st));
-            HIPCHK(f.to_tiles(o, out + k * f.tile_stride(), c.st));
+            Cx<TD> *o = out + k * f.spec_elems(s.FH);
+            HIPCHK(f.row_r2c(ROW_PLAIN, img.p, 0, s.FW, s.FW, nullptr, nullptr, 0, o, 0, s.FH, s.FH, 1, c.st));
+            HIPCHK(f.template colpass<TD>(COL_FWD, o, 0, s.FH, o, 0, s.FH, nullptr, 0, s.FH, s.FH, 1, c.st));
             HIPCHK(hipStreamSynchronize(c.st));
         }
     }
@@ -314,13 +311,14 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const FftShape &s = f.s;
         const int n = (int)c.ex.size();
         DevBuf<Cx<TD>> sk, sx;
-        sk.alloc((size_t)n * c.H * s.NCP);
-        sx.alloc((size_t)c.H * s.NCP);
-        HIPCHK(f.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, dex, wts, c.W, sk.p, (size_t)c.H * s.NCP, c.H, n, c.st));
-        HIPCHK(f.template colpass<TD>(COL_ADJ, sk.p, (size_t)c.H * s.NCP, sx.p, 0, pspec, f.tile_stride(), c.H,
-                                      c.H, n, c.st));
+        const size_t ks = f.spec_elems(c.H);
+        sk.alloc((size_t)n * ks);
+        sx.alloc(ks);
+        HIPCHK(f.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, dex, wts, c.W, sk.p, ks, c.H, c.H, n, c.st));
+        HIPCHK(f.template colpass<TD>(COL_ADJ, sk.p, ks, c.H, sx.p, 0, c.H, pspec, f.spec_elems(s.FH), c.H, c.H, n,
+                                      c.st));
         const TD sc = (TD)1 / (TD)((size_t)s.FW * s.FH);
-        HIPCHK(f.row_c2r(sx.p, 0, Dout, 0, c.W, c.W, sc, c.H, 1, c.st));
+        HIPCHK(f.row_c2r(sx.p, 0, c.H, Dout, 0, c.W, c.W, sc, c.H, 1, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
     }
 
@@ -423,7 +421,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         upload_ex(c);
 
         // PSF spectra
-        psf_spec.alloc((size_t)n * fft.tile_stride());
+        psf_spec.alloc((size_t)n * fft.spec_elems(s.FH));
         psf_spectra<TF>(c, fft, psf_spec.p);
 
         // initial drizzle (updc_core.c:905-983)
@@ -446,7 +444,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             if (!e2.empty()) throw Error(UPDC_ERR_ARG, e2);
             try {
                 DevBuf<Cx<double>> ps;
-                ps.alloc((size_t)n * f64.tile_stride());
+                ps.alloc((size_t)n * f64.spec_elems(f64.s.FH));
                 psf_spectra<double>(c, f64, ps.p);
                 DevBuf<double> wd;
                 wd.alloc(npix);
@@ -469,8 +467,8 @@ template <class TF, class TA> struct Pipe : PipeBase {
         // iteration buffers
         blur.alloc((size_t)n * nout);
         C.alloc(nout);
-        spec_x.alloc((size_t)std::max(c.H, s.FH) * s.NCP);
-        spec_k.alloc((size_t)n * c.H * s.NCP);
+        spec_x.alloc(fft.spec_elems(std::max(c.H, s.FH)));
+        spec_k.alloc((size_t)n * fft.spec_elems(c.H));
         HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, npix, c.st));
         diag_cur.alloc(n);
         idiag_cur.alloc(1);
@@ -494,35 +492,38 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const int n = N(c);
         const FftShape &s = fft.s;
         const size_t nout = c.nout();
-        const size_t kstride = (size_t)c.H * s.NCP;
+        const size_t kstride = fft.spec_elems(c.H), fstride = fft.spec_elems(s.FH);
         const TF sc = scale(c);
         auto mark = [&](int i) { if (ev) HIPCHK(hipEventRecord(ev[i], c.st)); };
         if (!spec_tmp.p) {
-            spec_tmp.alloc((size_t)s.FH * s.NCP);
-            spec_acc.alloc((size_t)s.FH * s.NCP);
+            spec_tmp.alloc(fstride);
+            spec_acc.alloc(fstride);
         }
-        Cx<TF> *xh = spec_x.p;  // FH rows after the column pass
+        Cx<TF> *xh = spec_x.p;  // full FH-row spectrum (pitch rp(FH)) after the column pass
         mark(0);
-        HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, xh, 0, c.H, 1, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_FWD, xh, 0, xh, 0, nullptr, 0, c.H, s.FH, 1, c.st));
-        HIPCHK(fill<TA>((TA *)spec_acc.p, (TA)0, 2 * (size_t)s.FH * s.NCP, c.st));
+        HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, xh, 0, s.FH, c.H, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_FWD, xh, 0, s.FH, xh, 0, s.FH, nullptr, 0, c.H, s.FH, 1, c.st));
+        HIPCHK(fill<TA>((TA *)spec_acc.p, (TA)0, 2 * fstride, c.st));
         mark(1);
         for (int k = 0; k < n; ++k) {
-            const Cx<TF> *pk = psf_spec.p + (size_t)k * fft.tile_stride();
+            const Cx<TF> *pk = psf_spec.p + (size_t)k * fstride;
             Cx<TF> *sk = spec_k.p + (size_t)k * kstride;
             HIPCHK(fft.spec_mul(xh, pk, spec_tmp.p, c.st));
-            HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, sk, 0, nullptr, 0, s.FH, c.H, 1, c.st));
-            HIPCHK(fft.row_c2r(sk, 0, blur.p + (size_t)k * nout, 0, c.W, c.W, sc, c.H, 1, c.st));
+            HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, s.FH, sk, 0, c.H, nullptr, 0, s.FH, c.H, 1,
+                                            c.st));
+            HIPCHK(fft.row_c2r(sk, 0, c.H, blur.p + (size_t)k * nout, 0, c.W, c.W, sc, c.H, 1, c.st));
             HIPCHK((gather_freeze<TF, TA>(d_ex.p, k, k + 1, (uint32_t)c.ex[k]->nin, blur.p, nout, sci.p, w.p,
                                           frozen.p, wr.p, diag_cur.p, idiag_cur.p, c.st)));
-            HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p + k, wr.p, c.W, sk, 0, c.H, 1, c.st));
-            HIPCHK(fft.template colpass<TA>(COL_FWD, sk, 0, spec_tmp.p, 0, nullptr, 0, c.H, s.FH, 1, c.st));
+            HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p + k, wr.p, c.W, sk, 0, c.H, c.H, 1, c.st));
+            HIPCHK(fft.template colpass<TA>(COL_FWD, sk, 0, c.H, spec_tmp.p, 0, s.FH, nullptr, 0, c.H, s.FH, 1,
+                                            c.st));
             HIPCHK(fft.template spec_acc<TA>(spec_tmp.p, pk, spec_acc.p, c.st));
         }
         mark(2);
         HIPCHK(fft.template spec_cast<TA>(spec_acc.p, spec_tmp.p, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, spec_x.p, 0, nullptr, 0, s.FH, c.H, 1, c.st));
-        HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, s.FH, spec_x.p, 0, c.H, nullptr, 0, s.FH, c.H, 1,
+                                        c.st));
+        HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
         mark(3);
         HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
         mark(4);
@@ -535,26 +536,26 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const int n = N(c);
         const FftShape &s = fft.s;
         const size_t nout = c.nout();
-        const size_t kstride = (size_t)c.H * s.NCP;
+        const size_t kstride = fft.spec_elems(c.H);
         const TF sc = scale(c);
         auto mark = [&](int i) { if (ev) HIPCHK(hipEventRecord(ev[i], c.st)); };
         mark(0);
-        HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
+        HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, c.H, 1, c.st));
         mark(1);
-        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, fft.tile_stride(),
-                                        c.H, c.H, n, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, c.H, spec_k.p, kstride, c.H, psf_spec.p,
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
         mark(2);
-        HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, sc, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, sc, c.H, n, c.st));
         mark(3);
         HIPCHK((gather_freeze<TF, TA>(d_ex.p, 0, n, max_nin, blur.p, nout, sci.p, w.p, frozen.p, wr.p, diag_cur.p,
                                       idiag_cur.p, c.st)));
         mark(4);
-        HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, n, c.st));
+        HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
         mark(5);
-        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, fft.tile_stride(),
-                                        c.H, c.H, n, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p,
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
         mark(6);
-        HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
+        HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
         mark(7);
         HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
         mark(8);
@@ -725,17 +726,20 @@ template <class TF, class TA> struct Pipe : PipeBase {
         if (est) to_host_f64(x.p, c.nout(), est, c);
         if (covo) to_host_f64(cov.p, c.nout(), covo, c);
     }
-    void get_psf_spectrum(Ctx &c, int k, double *out) override {
+    // column-major (pitch rp(rows)) device spectrum -> row-major rows x NC host array
+    void spectrum_to_host(Ctx &c, const Cx<TF> *d, int rows, double *out) {
         const FftShape &s = fft.s;
-        DevBuf<Cx<TF>> tmp;
-        tmp.alloc((size_t)s.FH * s.NCP);
-        HIPCHK(fft.from_tiles(psf_spec.p + (size_t)k * fft.tile_stride(), tmp.p, c.st));
-        HIPCHK(hipStreamSynchronize(c.st));
-        std::vector<double> row((size_t)s.NCP * 2);
-        for (int r = 0; r < s.FH; ++r) {
-            to_host_f64((const TF *)(tmp.p + (size_t)r * s.NCP), (size_t)s.NCP * 2, row.data(), c);
-            std::memcpy(out + (size_t)r * s.NC * 2, row.data(), (size_t)s.NC * 2 * sizeof(double));
-        }
+        const int P = rp(rows);
+        std::vector<double> cm((size_t)s.NC * P * 2);
+        to_host_f64((const TF *)d, cm.size(), cm.data(), c);
+        for (int v = 0; v < s.NC; ++v)
+            for (int r = 0; r < rows; ++r) {
+                out[((size_t)r * s.NC + v) * 2] = cm[((size_t)v * P + r) * 2];
+                out[((size_t)r * s.NC + v) * 2 + 1] = cm[((size_t)v * P + r) * 2 + 1];
+            }
+    }
+    void get_psf_spectrum(Ctx &c, int k, double *out) override {
+        spectrum_to_host(c, psf_spec.p + (size_t)k * fft.spec_elems(fft.s.FH), fft.s.FH, out);
     }
     void stage_forward(Ctx &c, const double *xin, double *bl) override {
         const int n = N(c);
@@ -744,11 +748,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
         DevBuf<TF> xs;
         xs.alloc(nout);
         from_host_f64(xin, nout, xs.p, c);
-        const size_t kstride = (size_t)c.H * s.NCP;
-        HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, fft.tile_stride(),
-                                        c.H, c.H, n, c.st));
-        HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
+        const size_t kstride = fft.spec_elems(c.H);
+        HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, c.H, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, c.H, spec_k.p, kstride, c.H, psf_spec.p,
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(blur.p, (size_t)n * nout, bl, c);
     }
@@ -759,11 +763,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
         DevBuf<TF> xs;
         xs.alloc(nout);
         from_host_f64(xin, nout, xs.p, c);
-        const size_t kstride = (size_t)c.H * s.NCP;
-        HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, 1, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, spec_k.p, kstride, psf_spec.p, fft.tile_stride(),
-                                        c.H, c.H, n, c.st));
-        HIPCHK(fft.row_c2r(spec_k.p, kstride, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
+        const size_t kstride = fft.spec_elems(c.H);
+        HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, c.H, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, c.H, spec_k.p, kstride, c.H, psf_spec.p,
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
         HIPCHK(predict<TF>(d_ex.p, n, max_nin, blur.p, nout, wr.p, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(wr.p, c.npix(), pred, c);
@@ -795,12 +799,12 @@ template <class TF, class TA> struct Pipe : PipeBase {
     void stage_adjoint(Ctx &c, const double *v, double *out) override {
         const int n = N(c);
         const FftShape &s = fft.s;
-        const size_t kstride = (size_t)c.H * s.NCP;
+        const size_t kstride = fft.spec_elems(c.H);
         from_host_f64(v, c.npix(), wr.p, c);
-        HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, n, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, spec_x.p, 0, psf_spec.p, fft.tile_stride(),
-                                        c.H, c.H, n, c.st));
-        HIPCHK(fft.row_c2r(spec_x.p, 0, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
+        HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p,
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(C.p, c.nout(), out, c);
     }
@@ -819,28 +823,28 @@ template <class TF, class TA> struct Pipe : PipeBase {
         im.alloc((size_t)s.FW * s.FH);
         from_host_f64(img, (size_t)s.FW * s.FH, im.p, c);
         DevBuf<Cx<TF>> sp;
-        sp.alloc((size_t)s.FH * s.NCP);
-        HIPCHK(fft.row_r2c(ROW_PLAIN, im.p, 0, s.FW, s.FW, nullptr, nullptr, 0, sp.p, 0, s.FH, 1, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_FWD, sp.p, 0, sp.p, 0, nullptr, 0, s.FH, s.FH, 1, c.st));
+        sp.alloc(fft.spec_elems(s.FH));
+        HIPCHK(fft.row_r2c(ROW_PLAIN, im.p, 0, s.FW, s.FW, nullptr, nullptr, 0, sp.p, 0, s.FH, s.FH, 1, c.st));
+        HIPCHK(fft.template colpass<TA>(COL_FWD, sp.p, 0, s.FH, sp.p, 0, s.FH, nullptr, 0, s.FH, s.FH, 1, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
-        std::vector<double> row((size_t)s.NCP * 2);
-        for (int r = 0; r < s.FH; ++r) {
-            to_host_f64((const TF *)(sp.p + (size_t)r * s.NCP), (size_t)s.NCP * 2, row.data(), c);
-            std::memcpy(spec + (size_t)r * s.NC * 2, row.data(), (size_t)s.NC * 2 * sizeof(double));
-        }
+        spectrum_to_host(c, sp.p, s.FH, spec);
     }
     void fft_c2r(Ctx &c, const double *spec, double *img) override {
         const FftShape &s = fft.s;
-        std::vector<double> padded((size_t)s.FH * s.NCP * 2, 0.0);
+        const int P = rp(s.FH);
+        std::vector<double> cm((size_t)s.NC * P * 2, 0.0);
         for (int r = 0; r < s.FH; ++r)
-            std::memcpy(padded.data() + (size_t)r * s.NCP * 2, spec + (size_t)r * s.NC * 2, (size_t)s.NC * 2 * 8);
+            for (int v = 0; v < s.NC; ++v) {
+                cm[((size_t)v * P + r) * 2] = spec[((size_t)r * s.NC + v) * 2];
+                cm[((size_t)v * P + r) * 2 + 1] = spec[((size_t)r * s.NC + v) * 2 + 1];
+            }
         DevBuf<Cx<TF>> sp;
-        sp.alloc((size_t)s.FH * s.NCP);
-        from_host_f64(padded.data(), padded.size(), (TF *)sp.p, c);
-        HIPCHK(fft.template colpass<TA>(COL_INV, sp.p, 0, sp.p, 0, nullptr, 0, s.FH, s.FH, 1, c.st));
+        sp.alloc(fft.spec_elems(s.FH));
+        from_host_f64(cm.data(), cm.size(), (TF *)sp.p, c);
+        HIPCHK(fft.template colpass<TA>(COL_INV, sp.p, 0, s.FH, sp.p, 0, s.FH, nullptr, 0, s.FH, s.FH, 1, c.st));
         DevBuf<TF> im;
         im.alloc((size_t)s.FW * s.FH);
-        HIPCHK(fft.row_c2r(sp.p, 0, im.p, 0, s.FW, s.FW, (TF)1, s.FH, 1, c.st));
+        HIPCHK(fft.row_c2r(sp.p, 0, s.FH, im.p, 0, s.FW, s.FW, (TF)1, s.FH, 1, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(im.p, (size_t)s.FW * s.FH, img, c);
     }
@@ -1264,6 +1268,33 @@ int updc_fft_r2c_f64(updc_ctx *ctx, const double *image, double *spectrum) {
 int updc_fft_c2r_f64(updc_ctx *ctx, const double *spectrum, double *image) {
     return guarded([&] { NEED_PREPARED(ctx); need(image && spectrum, UPDC_ERR_ARG, "null"); ctx->pipe->fft_c2r(*ctx, spectrum, image); });
 }
+int updc_copy_calibration(size_t bytes, int reps, int device, double *ms) {
+    return guarded([&] {
+        need(bytes >= 16 && reps >= 1, UPDC_ERR_ARG, "invalid copy size");
+        HIPCHK(hipSetDevice(device));
+        DevBuf<uint8_t> a, b;
+        a.alloc(bytes);
+        b.alloc(bytes);
+        hipStream_t st;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(fill<uint8_t>(a.p, (uint8_t)1, bytes, st));
+        HIPCHK(copy_ref(a.p, b.p, bytes, st));
+        HIPCHK(hipEventRecord(e0, st));
+        for (int i = 0; i < reps; ++i) HIPCHK(copy_ref(a.p, b.p, bytes, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        if (ms) *ms = t / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(st);
+    });
+}
+
 int updc_get_stage_times(updc_ctx *ctx, char names[][32], double *ms, int *launches, int *count) {
     return guarded([&] {
         NEED_PREPARED(ctx);

@@ -841,6 +841,16 @@ template <class TF> hipError_t csc_apply(const ExpDev *e_dev, const TF *v, size_
     return hipGetLastError();
 }
 
+// Streaming copy of known size: the scale reference for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy_ref(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+hipError_t copy_ref(const void *src, void *dst, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_ref, dim3(8192), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n);
+    return hipGetLastError();
+}
+
 // ---- explicit instantiations
 template hipError_t csc_apply<float>(const ExpDev *, const float *, size_t, float *, hipStream_t);
 template hipError_t csc_apply<double>(const ExpDev *, const double *, size_t, double *, hipStream_t);

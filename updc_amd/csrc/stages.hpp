@@ -75,6 +75,7 @@ template <class TF>
 hipError_t psf_wrap(const float *psf, int pw, int ph, int transposed, int FW, int FH, TF inv_sum, TF *image,
                     hipStream_t st);
 template <class TS, class TD> hipError_t convert(const TS *s, TD *d, size_t n, hipStream_t st);
+hipError_t copy_ref(const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t init_diag(ExpDiag *d, int n, IterDiag *it, int nit, hipStream_t st);
 // Iteration bookkeeping that stays inside a captured graph: copy the current
 // diagnostics to history slot *ctr, advance *ctr and clear the current slot.
