@@ -156,6 +156,10 @@ static size_t linkmap_bytes(uint64_t nin, uint64_t nlinks, uint32_t enc) {
     return 0;
 }
 
+// The gather stages links with aligned 16-byte loads, whose last block may
+// extend up to 15 bytes past the image: allocate whole 16-byte blocks.
+static size_t lm_alloc_bytes(size_t bytes) { return (bytes + 15) & ~(size_t)15; }
+
 struct HostExp {
     int iw = 0, ih = 0;
     size_t nin = 0, pix = 0;
@@ -913,7 +917,7 @@ static void add_linkmap_image(Ctx &c, HostExp &e, const void *img, size_t bytes)
     std::memcpy(&e.encoding, (const uint8_t *)img + 8, 4);
     std::memcpy(&e.nlinks, (const uint8_t *)img + 24, 8);
     e.lm_bytes = bytes;
-    e.lm.alloc(bytes);
+    e.lm.alloc(lm_alloc_bytes(bytes));
     HIPCHK(hipMemcpy(e.lm.p, img, bytes, hipMemcpyHostToDevice));
     if (e.encoding != UPDC_LM_DELTA16 && e.encoding != UPDC_LM_ABS32) throw Error(UPDC_ERR_FORMAT, "invalid LinkMap encoding");
     // the offsets are checked on the device before any kernel follows them
@@ -960,7 +964,7 @@ static void build_linkmap(Ctx &c, HostExp &e, const double *vert) {
     e.encoding = (fl & 1u) ? UPDC_LM_ABS32 : UPDC_LM_DELTA16;
     e.nlinks = total;
     e.lm_bytes = linkmap_bytes(e.nin, total, e.encoding);
-    e.lm.alloc(e.lm_bytes);
+    e.lm.alloc(lm_alloc_bytes(e.lm_bytes));
     HIPCHK(hipMemsetAsync(e.lm.p, 0, e.lm_bytes, c.st));
     uint32_t h[16] = {0};
     h[0] = UPDC_LM_MAGIC; h[1] = UPDC_LM_VERSION; h[2] = e.encoding; h[3] = 64;

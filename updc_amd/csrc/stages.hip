@@ -459,13 +459,22 @@ hipError_t drizzle(const ExpDev *ex, int nexp, const TF *sci, const TF *w, int n
 // the reference.  MIXED: pred in f32, raw_mu, the test and the ratio in f64.
 // F64: everything in f64.
 // Grid-stride over chunks of 256 consecutive pixels of one exposure
-// (blockIdx.y).  A chunk owns one contiguous link range [off[p0], off[p0+256]),
-// staged in LDS with coalesced loads before each lane walks its own links
-// from LDS; chunks whose range exceeds the LDS capacity read their links from
-// HBM directly.  FreezeMu counters and the minimum raw_mu are reduced per
-// block and committed with one atomic per block (a per-wave atomic on the
-// exposure's counters serialises the whole launch).
-static constexpr int GATHER_NT = 256, GATHER_CAP = 4096, GATHER_BLOCKS = 1280, GATHER_UNROLL = 12;
+// (blockIdx.y).  A chunk owns one contiguous link range [off[p0], off[p0+256]).
+// Each chunk costs two dependent memory round trips: (1) the chunk's link
+// words and fractions, fetched as aligned 16-byte blocks and issued together
+// with the lanes' own pixel operands (off, base, w, sci, frozen); (2) the blur
+// gathers, addressed from LDS.  The next chunk's link range is fetched one
+// chunk ahead.  Chunks whose link range exceeds the LDS capacity read their
+// links from HBM directly.  FreezeMu counters and the minimum raw_mu are
+// reduced per block and committed with one atomic per block (a per-wave
+// atomic on the exposure's counters serialises the whole launch).
+// GATHER_CAP keeps the block's LDS under 32 KiB, five blocks per CU; the
+// grid is one wave of five blocks on each of the 256 CUs.
+static constexpr int GATHER_NT = 256, GATHER_CAP = 4000, GATHER_BLOCKS = 1280, GATHER_UNROLL = 12;
+// 16-byte blocks of LDS for link words (u16 deltas or u32 targets) and for
+// fractions: GATHER_CAP links plus one block of misalignment slack each.
+static constexpr int GATHER_LV = GATHER_CAP / 4 + 1, GATHER_FV = GATHER_CAP / 4 + 1;
+static constexpr int GATHER_SLOTS = (GATHER_LV + GATHER_NT - 1) / GATHER_NT;
 
 template <class TF, class TA>
 __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__ ex, int kbeg,
@@ -473,56 +482,103 @@ __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__
                                                       const TF *__restrict__ sci, const TF *__restrict__ w,
                                                       uint8_t *__restrict__ frozen, TF *__restrict__ wr,
                                                       ExpDiag *diag, IterDiag *idiag) {
-    __shared__ uint32_t s_tgt[GATHER_CAP];
-    __shared__ float s_frac[GATHER_CAP];
+    __shared__ uint4 s_lv[GATHER_LV];
+    __shared__ uint4 s_fv[GATHER_FV];
     __shared__ unsigned int r_new[GATHER_NT / 64], r_tot[GATHER_NT / 64];
     __shared__ TA r_min[GATHER_NT / 64];
     const int k = kbeg + blockIdx.y;
     const ExpDev &e = ex[k];
+    const uint32_t nin = e.nin;
     const bool d16 = e.encoding == UPDC_LM_DELTA16;
+    const uintptr_t lbase = d16 ? (uintptr_t)e.d16 : (uintptr_t)e.t32;
+    const uint32_t lsz = d16 ? 2u : 4u;
+    const uint32_t *__restrict__ off = e.off;
+    const float *__restrict__ frac = e.frac;
     const TF *bl = blur + (size_t)k * blur_bstride;
+    const uint16_t *s16 = (const uint16_t *)s_lv;
+    const uint32_t *s32 = (const uint32_t *)s_lv;
+    const float *sfr = (const float *)s_fv;
     unsigned int nnew = 0, ntot = 0, bad = 0;
     unsigned long long badidx = ~0ull;
     TA mn = (TA)INFINITY;
-    for (uint32_t p0 = blockIdx.x * GATHER_NT; p0 < e.nin; p0 += gridDim.x * GATHER_NT) {
+    const uint32_t step = gridDim.x * GATHER_NT;
+    uint32_t p0 = blockIdx.x * GATHER_NT;
+    uint32_t Z0 = 0, Z1 = 0;
+    if (p0 < nin) {
+        Z0 = off[p0];
+        Z1 = off[min(p0 + (uint32_t)GATHER_NT, nin)];
+    }
+    for (; p0 < nin; p0 += step) {
         const uint32_t p = p0 + threadIdx.x;
-        const uint32_t pend = min(p0 + (uint32_t)GATHER_NT, e.nin);
-        const uint32_t Z0 = e.off[p0], Z1 = e.off[pend];
-        const bool staged = Z1 - Z0 <= (uint32_t)GATHER_CAP;
+        const uint32_t pend = min(p0 + (uint32_t)GATHER_NT, nin);
+        // aligned 16-byte block ranges covering the chunk's links and fractions
+        const uintptr_t la = lbase + (uintptr_t)Z0 * lsz, fa = (uintptr_t)(frac + Z0);
+        const uintptr_t la0 = la & ~(uintptr_t)15, fa0 = fa & ~(uintptr_t)15;
+        const uint32_t nlv = (uint32_t)((((lbase + (uintptr_t)Z1 * lsz) + 15) & ~(uintptr_t)15) - la0) >> 4;
+        const uint32_t nfv = (uint32_t)((((uintptr_t)(frac + Z1) + 15) & ~(uintptr_t)15) - fa0) >> 4;
+        const bool staged = nlv <= (uint32_t)GATHER_LV && nfv <= (uint32_t)GATHER_FV;
+        const uint32_t lsh = (uint32_t)(la - la0) / lsz, fsh = (uint32_t)(fa - fa0) >> 2;
+        // round trip 1: staging blocks and the lane's pixel operands together
+        const uint32_t nl = staged ? nlv : 0u, nf = staged ? nfv : 0u;
+        uint4 lv[GATHER_SLOTS], fv[GATHER_SLOTS];
+#pragma unroll
+        for (int s = 0; s < GATHER_SLOTS; ++s) {
+            const uint32_t i = threadIdx.x + s * GATHER_NT;
+            lv[s] = i < nl ? ((const uint4 *)la0)[i] : uint4{};
+            fv[s] = i < nf ? ((const uint4 *)fa0)[i] : uint4{};
+        }
+        uint32_t z0 = 0, z1 = 0, b0 = 0;
+        TF wp = (TF)0, sv = (TF)0;
+        uint8_t fz = 0;
+        const size_t gp = e.pix + p;
+        if (p < pend) {
+            z0 = off[p];
+            z1 = off[p + 1];
+            if (d16) b0 = e.base[p];
+            wp = w[gp];
+            sv = sci[gp];
+            fz = frozen[gp];
+        }
+        // the next chunk's link range
+        const uint32_t pn = p0 + step;
+        uint32_t Zn0 = 0, Zn1 = 0;
+        if (pn < nin) {
+            Zn0 = off[pn];
+            Zn1 = off[min(pn + (uint32_t)GATHER_NT, nin)];
+        }
         __syncthreads();  // previous chunk's LDS reads are done
-        if (staged) {
-            // coalesced copies; DELTA16 keeps the raw delta, the lane adds its base
-            if (d16)
-                for (uint32_t z = Z0 + threadIdx.x; z < Z1; z += GATHER_NT) s_tgt[z - Z0] = (uint32_t)e.d16[z];
-            else
-                for (uint32_t z = Z0 + threadIdx.x; z < Z1; z += GATHER_NT) s_tgt[z - Z0] = e.t32[z];
-            for (uint32_t z = Z0 + threadIdx.x; z < Z1; z += GATHER_NT) s_frac[z - Z0] = e.frac[z];
+#pragma unroll
+        for (int s = 0; s < GATHER_SLOTS; ++s) {
+            const uint32_t i = threadIdx.x + s * GATHER_NT;
+            if (i < nl) s_lv[i] = lv[s];
+            if (i < nf) s_fv[i] = fv[s];
         }
         __syncthreads();
         if (p < pend) {
-            const size_t gp = e.pix + p;
-            const TF wp = w[gp];
-            const uint32_t z0 = e.off[p], z1 = e.off[p + 1];
             TF out = (TF)0;
             if (wp > (TF)0 && z0 != z1) {
                 TF pred = (TF)0;
-                const uint32_t b0 = d16 ? e.base[p] : 0u;
                 if (staged) {
-                    // issue the first GATHER_UNROLL blur loads together, then run the
-                    // fmaf chain in link order (same operations, one memory round trip)
-                    const uint32_t nz = z1 - z0, l0 = z0 - Z0;
+                    // round trip 2: the first GATHER_UNROLL blur loads issued together,
+                    // then the fmaf chain in link order (same operations as a plain loop)
+                    const uint32_t nz = z1 - z0, l0 = z0 - Z0 + lsh, f0 = z0 - Z0 + fsh;
                     TF vals[GATHER_UNROLL];
 #pragma unroll
-                    for (int j = 0; j < GATHER_UNROLL; ++j) vals[j] = (uint32_t)j < nz ? bl[b0 + s_tgt[l0 + j]] : (TF)0;
+                    for (int j = 0; j < GATHER_UNROLL; ++j) {
+                        vals[j] = (TF)0;
+                        if ((uint32_t)j < nz) vals[j] = bl[d16 ? b0 + (uint32_t)s16[l0 + j] : s32[l0 + j]];
+                    }
 #pragma unroll
                     for (int j = 0; j < GATHER_UNROLL; ++j)
-                        if ((uint32_t)j < nz) pred = fmat((TF)s_frac[l0 + j], vals[j], pred);
-                    for (uint32_t j = GATHER_UNROLL; j < nz; ++j)
-                        pred = fmat((TF)s_frac[l0 + j], bl[b0 + s_tgt[l0 + j]], pred);
+                        if ((uint32_t)j < nz) pred = fmat((TF)sfr[f0 + j], vals[j], pred);
+                    for (uint32_t j = GATHER_UNROLL; j < nz; ++j) {
+                        const uint32_t t = d16 ? b0 + (uint32_t)s16[l0 + j] : s32[l0 + j];
+                        pred = fmat((TF)sfr[f0 + j], bl[t], pred);
+                    }
                 } else if (d16) {
-                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)e.frac[z], bl[b0 + (uint32_t)e.d16[z]], pred);
+                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)frac[z], bl[b0 + (uint32_t)e.d16[z]], pred);
                 } else {
-                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)e.frac[z], bl[e.t32[z]], pred);
+                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)frac[z], bl[e.t32[z]], pred);
                 }
                 TA raw;
                 if constexpr (sizeof(TA) == 4) raw = pred + e.mean_offset_f;
@@ -530,7 +586,6 @@ __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__
                 unsigned int b = 0;
                 if (!isfinite(raw)) b |= UPDC_E_MU;
                 mn = raw < mn ? raw : mn;
-                uint8_t fz = frozen[gp];
                 if (!fz && !(raw > (TA)0)) {
                     fz = 1;
                     frozen[gp] = 1;
@@ -542,8 +597,8 @@ __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__
                     ratio = (TA)1;
                 } else {
                     TA y;
-                    if constexpr (sizeof(TA) == 4) y = sci[gp] + e.pedestal_f;
-                    else y = (TA)sci[gp] + (TA)e.pedestal;
+                    if constexpr (sizeof(TA) == 4) y = sv + e.pedestal_f;
+                    else y = (TA)sv + (TA)e.pedestal;
                     ratio = y / raw;
                     if (!isfinite(ratio)) b |= UPDC_E_RATIO;
                 }
@@ -555,6 +610,8 @@ __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__
             }
             wr[gp] = out;
         }
+        Z0 = Zn0;
+        Z1 = Zn1;
     }
     // block reduction, one atomic per block and quantity
     for (int o = 32; o >= 1; o >>= 1) {
