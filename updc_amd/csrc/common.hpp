@@ -254,6 +254,10 @@ __device__ __forceinline__ void edge_last(Cx<T> (&rb)[NBE][RE], const Cx<T> *buf
             bfly<RE, INV>(v);
 #pragma unroll
             for (int q = 0; q < RE; ++q) rb[r][q] = v[q];
+        } else {
+            // a full definition, so the registers are dead between the edges
+#pragma unroll
+            for (int q = 0; q < RE; ++q) rb[r][q] = cx<T>((T)0, (T)0);
         }
     }
     __syncthreads();

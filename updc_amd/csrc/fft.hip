@@ -17,30 +17,9 @@
 namespace updc {
 
 // ------------------------------------------------------------------ kernels
-template <class T> __device__ __forceinline__ T csc_gather(const ExpDev &e, const T *__restrict__ w, uint32_t t) {
-    // scatter[t] = sum over source pixels p (ascending) of fma(frac, w[p], .)
-    // from 0 -- the same fmaf chain as updc_core.c:1173-1176 / :1064, since the
-    // reference visits pixels in ascending p and each (p,t) pair once.
-    const uint32_t a = e.toff[t];
-    const uint32_t n = e.toff[t + 1] - a;
-    // entries and their w loads issued together, then the chain in order
-    constexpr int U = 6;
-    uint2 en[U];
-    T wv[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) en[j] = (uint32_t)j < n ? e.tent[a + j] : make_uint2(0u, 0u);
-#pragma unroll
-    for (int j = 0; j < U; ++j) wv[j] = (uint32_t)j < n ? w[en[j].x] : (T)0;
-    T acc = (T)0;
-#pragma unroll
-    for (int j = 0; j < U; ++j)
-        if ((uint32_t)j < n) acc = fmat((T)__uint_as_float(en[j].y), wv[j], acc);
-    for (uint32_t j = U; j < n; ++j) {
-        const uint2 x = e.tent[a + j];
-        acc = fmat((T)__uint_as_float(x.y), w[x.x], acc);
-    }
-    return acc;
-}
+// Row gather batching: RG_GB targets per lane per batch, the first RG_GU CSC
+// entries of each loaded together (further entries, rare, one at a time).
+static constexpr int RG_GB = 8, RG_GU = 3;
 
 // Row R2C: RB consecutive rows per workgroup (interleaved in LDS, row j at
 // slot (n << LG) + j), one half-length complex FFT per row, then the real
@@ -55,10 +34,10 @@ __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restric
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
     const int r0 = blockIdx.x * RB, b = blockIdx.y, M = f.L;
+    if (MODE == ROW_PLAIN) {
 #pragma unroll
-    for (int j = 0; j < RB; ++j) {
-        const int r = r0 + j;
-        if (MODE == ROW_PLAIN) {
+        for (int j = 0; j < RB; ++j) {
+            const int r = r0 + j;
             const T *row = in + (size_t)b * in_bstride + (size_t)r * in_pitch;
             for (int n = threadIdx.x; n < M; n += NT) {
                 const int c = 2 * n;
@@ -66,15 +45,57 @@ __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restric
                 const T x1 = (r < rows && c + 1 < win) ? row[c + 1] : (T)0;
                 z[(n << LG) + j] = cx<T>(x0, x1);
             }
-        } else {
-            const ExpDev &e = ex[b];
-            const T *w = wr + e.pix;
-            const uint32_t t0 = (uint32_t)r * (uint32_t)W;
-            for (int n = threadIdx.x; n < M; n += NT) {
-                const int c = 2 * n;
-                const T x0 = (r < rows && c < W) ? csc_gather<T>(e, w, t0 + c) : (T)0;
-                const T x1 = (r < rows && c + 1 < W) ? csc_gather<T>(e, w, t0 + c + 1) : (T)0;
-                z[(n << LG) + j] = cx<T>(x0, x1);
+        }
+    } else {
+        // Target column c of row j is the real (c even) or imaginary part of
+        // complex slot ((c/2) << LG) + j.  Lanes take consecutive targets; a
+        // batch of GB targets per lane issues its offset loads, then its
+        // entry loads, then its w loads together (three round trips per batch).
+        const ExpDev &e = ex[b];
+        const T *w = wr + e.pix;
+        T *zf = reinterpret_cast<T *>(z);
+        const int wc = min(W, 2 * M);
+        for (int j = 0; j < RB; ++j)
+            for (int c = wc + threadIdx.x; c < 2 * M; c += NT) zf[2 * (((c >> 1) << LG) + j) + (c & 1)] = (T)0;
+        const int per_row = (wc + NT - 1) / NT, total = RB * per_row;
+        for (int u0 = 0; u0 < total; u0 += RG_GB) {
+            uint32_t a[RG_GB], cnt[RG_GB];
+            int slot[RG_GB];
+#pragma unroll
+            for (int g = 0; g < RG_GB; ++g) {
+                const int u = u0 + g, j = u / per_row, c = threadIdx.x + (u - j * per_row) * NT, r = r0 + j;
+                slot[g] = (u < total && c < wc) ? 2 * (((c >> 1) << LG) + j) + (c & 1) : -1;
+                const bool ok = slot[g] >= 0 && r < rows;
+                const uint32_t t = (uint32_t)r * (uint32_t)W + (uint32_t)c;
+                a[g] = ok ? e.toff[t] : 0u;
+                cnt[g] = ok ? e.toff[t + 1] : 0u;
+            }
+            uint2 en[RG_GB][RG_GU];
+            T wv[RG_GB][RG_GU];
+#pragma unroll
+            for (int g = 0; g < RG_GB; ++g) {
+                cnt[g] -= a[g];
+#pragma unroll
+                for (int k = 0; k < RG_GU; ++k) en[g][k] = (uint32_t)k < cnt[g] ? e.tent[a[g] + k] : make_uint2(0u, 0u);
+            }
+#pragma unroll
+            for (int g = 0; g < RG_GB; ++g)
+#pragma unroll
+                for (int k = 0; k < RG_GU; ++k) wv[g][k] = (uint32_t)k < cnt[g] ? w[en[g][k].x] : (T)0;
+#pragma unroll
+            for (int g = 0; g < RG_GB; ++g) {
+                // scatter[t] = sum over source pixels p (ascending) of fma(frac, w[p], .)
+                // from 0 -- the fmaf chain of updc_core.c:1173-1176 / :1064, since
+                // the reference visits pixels in ascending p and each (p,t) pair once
+                T acc = (T)0;
+#pragma unroll
+                for (int k = 0; k < RG_GU; ++k)
+                    if ((uint32_t)k < cnt[g]) acc = fmat((T)__uint_as_float(en[g][k].y), wv[g][k], acc);
+                for (uint32_t k = RG_GU; k < cnt[g]; ++k) {
+                    const uint2 x = e.tent[a[g] + k];
+                    acc = fmat((T)__uint_as_float(x.y), w[x.x], acc);
+                }
+                if (slot[g] >= 0) zf[slot[g]] = acc;
             }
         }
     }
@@ -230,6 +251,122 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
     }
 }
 
+// Column pass with register-resident edge stages, for column plans whose
+// first and last radix are both RE (LR = FH / RE).  Element i + q LR (i < LR)
+// of a column is value q of butterfly i: the positions the first Stockham
+// stage reads and the last one writes.  Thread tid holds butterflies
+// tid + r NT.  The column loads therefore feed the first stage directly, the
+// last stage leaves the natural-order spectrum in registers, where the PSF
+// product and the stores use the same coalesced pattern, and an inverse
+// transform that follows starts from those registers -- two LDS passes and
+// their barriers fewer per transform than the LDS-only kernel.  The
+// butterflies, twiddles and their order are those of the LDS-only kernel.
+// PF: the next PSF column (BCAST) or the current one (ADJ) is loaded into
+// registers before the transform that precedes its use.
+static constexpr int EDGE_V = 16;  // complex values per thread per register set
+template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF>
+__global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__restrict__ in, size_t in_bs, int in_rp,
+                                              Cx<T> *__restrict__ out, size_t out_bs, int out_rp,
+                                              const Cx<T> *__restrict__ psf, size_t psf_bs, int psf_rp, int hin,
+                                              int hout, int nexp) {
+    constexpr int NBE = EDGE_V / RE;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
+    const int col = xcd_tile(blockIdx.x, nc);
+    if (col < 0) return;
+    const int LR = f.L / RE, ns = f.nst, tid = threadIdx.x;
+    const Cx<T> *twl = (const Cx<T> *)f.tw + f.to[ns - 1];
+    Cx<T> rb[NBE][RE];
+
+    auto load = [&](Cx<T>(&d)[NBE][RE], const Cx<T> *c, int h) {
+#pragma unroll
+        for (int r = 0; r < NBE; ++r)
+#pragma unroll
+            for (int q = 0; q < RE; ++q) {
+                const int i = tid + r * NT, n = i + q * LR;
+                d[r][q] = (i < LR && n < h) ? c[n] : cx<T>((T)0, (T)0);
+            }
+    };
+    auto store = [&](Cx<T> *c) {
+#pragma unroll
+        for (int r = 0; r < NBE; ++r)
+#pragma unroll
+            for (int q = 0; q < RE; ++q) {
+                const int i = tid + r * NT, n = i + q * LR;
+                if (i < LR && n < hout) c[n] = rb[r][q];
+            }
+    };
+    auto fwd = [&]() {
+        edge_first<RE, false, T, NT, NBE>(rb, buf, LR);
+        fft_lds_range<false, T, NT, VMAX>(buf, f, 1, ns - 1);
+        edge_last<RE, false, T, NT, NBE>(rb, buf, LR, twl);
+    };
+    auto inv = [&]() {
+        edge_first<RE, true, T, NT, NBE>(rb, buf, LR);
+        fft_lds_range<true, T, NT, VMAX>(buf, f, 1, ns - 1);
+        edge_last<RE, true, T, NT, NBE>(rb, buf, LR, twl);
+    };
+
+    if (MODE == COL_FWD || MODE == COL_INV) {
+        const int b = blockIdx.y;
+        load(rb, in + (size_t)b * in_bs + (size_t)col * in_rp, hin);
+        if (MODE == COL_FWD) fwd();
+        else inv();
+        store(out + (size_t)b * out_bs + (size_t)col * out_rp);
+    } else if (MODE == COL_BCAST) {
+        load(rb, in + (size_t)col * in_rp, hin);
+        Cx<T> pf[NBE][RE];
+        if (PF && nexp > 0) load(pf, psf + (size_t)col * psf_rp, f.L);
+        fwd();
+        Cx<T> xr[NBE][RE];
+#pragma unroll
+        for (int r = 0; r < NBE; ++r)
+#pragma unroll
+            for (int q = 0; q < RE; ++q) xr[r][q] = rb[r][q];
+        for (int k = 0; k < nexp; ++k) {
+            if (!PF) load(pf, psf + (size_t)k * psf_bs + (size_t)col * psf_rp, f.L);
+            // (a+ib)(c+id) as a*c-b*d, a*d+b*c (updc_core.c:756-758, adjoint=0)
+#pragma unroll
+            for (int r = 0; r < NBE; ++r)
+#pragma unroll
+                for (int q = 0; q < RE; ++q) rb[r][q] = cmul(xr[r][q], pf[r][q]);
+            if (PF && k + 1 < nexp) load(pf, psf + (size_t)(k + 1) * psf_bs + (size_t)col * psf_rp, f.L);
+            inv();
+            store(out + (size_t)k * out_bs + (size_t)col * out_rp);
+        }
+    } else {  // COL_ADJ
+        Cx<TA> acc[NBE][RE];
+#pragma unroll
+        for (int r = 0; r < NBE; ++r)
+#pragma unroll
+            for (int q = 0; q < RE; ++q) acc[r][q] = cx<TA>((TA)0, (TA)0);
+        for (int k = 0; k < nexp; ++k) {
+            load(rb, in + (size_t)k * in_bs + (size_t)col * in_rp, hin);
+            Cx<T> pf[NBE][RE];
+            if (PF) load(pf, psf + (size_t)k * psf_bs + (size_t)col * psf_rp, f.L);
+            fwd();
+            if (!PF) load(pf, psf + (size_t)k * psf_bs + (size_t)col * psf_rp, f.L);
+#pragma unroll
+            for (int r = 0; r < NBE; ++r)
+#pragma unroll
+                for (int q = 0; q < RE; ++q) {
+                    const Cx<T> p = pf[r][q], sv = rb[r][q];
+                    // re += a*c + b*d ; im += b*c - a*d  (updc_core.c:991-993)
+                    const T pre = sv.re * p.re + sv.im * p.im;
+                    const T pim = sv.im * p.re - sv.re * p.im;
+                    acc[r][q].re += (TA)pre;
+                    acc[r][q].im += (TA)pim;
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < NBE; ++r)
+#pragma unroll
+            for (int q = 0; q < RE; ++q) rb[r][q] = cx<T>((T)acc[r][q].re, (T)acc[r][q].im);
+        inv();
+        store(out + (size_t)col * out_rp);
+    }
+}
+
 // Elementwise spectrum kernels of the per-exposure form (the reference's
 // ws_multiply_spectrum / accumulate_adjoint_spectrum, updc_core.c:753-760,
 // 988-995), over full column-major spectra of pitch RP.
@@ -264,19 +401,44 @@ template <class T> struct RowCfg;
 template <> struct RowCfg<float> { static constexpr int NT = 1024, VMAX = 16, RB = 4; };
 template <> struct RowCfg<double> { static constexpr int NT = 512, VMAX = 16, RB = 2; };
 template <class T> struct ColCfg;
-template <> struct ColCfg<float> { static constexpr int NT = 512, VMAX = 16, XMAX = 16; };
-template <> struct ColCfg<double> { static constexpr int NT = 512, VMAX = 16, XMAX = 12; };
+// PF: PSF prefetch in the edge column kernel (FP64 would spill with it).
+template <> struct ColCfg<float> { static constexpr int NT = 512, VMAX = 16, XMAX = 16; static constexpr bool PF = true; };
+template <> struct ColCfg<double> { static constexpr int NT = 512, VMAX = 16, XMAX = 12; static constexpr bool PF = false; };
 static constexpr size_t LDS_BUDGET = 150 * 1024;
 
-static bool factor_radices(int L, FftLen &f) {
+// Radix order: 8s, a 4 or 2, then 7s, 5s, 3s.  With `edge`, a radix that
+// occurs twice (preferring 8, 5, 4, 7, 3) is moved to both ends, so the
+// first and last stages can run in registers; returns that radix in *re
+// (0 when none repeats).
+static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullptr) {
     int n = L, k = 0;
-    auto push = [&](int r) { if (k < UPDC_MAX_STAGES) f.rad[k++] = r; };
+    int rad[UPDC_MAX_STAGES + 8];
+    auto push = [&](int r) { if (k < UPDC_MAX_STAGES) rad[k++] = r; };
     while (n % 8 == 0) { push(8); n /= 8; }
     if (n % 4 == 0) { push(4); n /= 4; }
     if (n % 2 == 0) { push(2); n /= 2; }
     while (n % 7 == 0) { push(7); n /= 7; }
     while (n % 5 == 0) { push(5); n /= 5; }
     while (n % 3 == 0) { push(3); n /= 3; }
+    if (re) *re = 0;
+    if (edge && k >= 2) {
+        for (int cand : {8, 5, 4, 7, 3}) {
+            int cnt = 0;
+            for (int i = 0; i < k; ++i) cnt += rad[i] == cand;
+            if (cnt < 2) continue;
+            int mid[UPDC_MAX_STAGES], m = 0, skip = 2;
+            for (int i = 0; i < k; ++i) {
+                if (rad[i] == cand && skip > 0) { --skip; continue; }
+                mid[m++] = rad[i];
+            }
+            rad[0] = cand;
+            for (int i = 0; i < m; ++i) rad[1 + i] = mid[i];
+            rad[k - 1] = cand;
+            if (re) *re = cand;
+            break;
+        }
+    }
+    for (int i = 0; i < k; ++i) f.rad[i] = rad[i];
     f.L = L;
     f.nst = k;
     long ns = 1;
@@ -358,11 +520,29 @@ template <class T, class TA, int MODE> static void *col_fn() {
     return (void *)k_col<T, TA, ColCfg<T>::NT, ColCfg<T>::VMAX, ColCfg<T>::XMAX, MODE>;
 }
 
+template <class T, class TA, int MODE, bool PF> static void *col_e_fn_pf(int re) {
+    constexpr int NT = ColCfg<T>::NT, V = ColCfg<T>::VMAX;
+    switch (re) {
+        case 3: return (void *)k_col_e<T, TA, NT, V, 3, MODE, PF>;
+        case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF>;
+        case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF>;
+        case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF>;
+        case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF>;
+        default: return nullptr;
+    }
+}
+template <class T, class TA, int MODE> static void *col_e_fn(int re) {
+    constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ);
+    return col_e_fn_pf<T, TA, MODE, PF>(re);
+}
+
 template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (FW < 2 || FH < 1 || (FW & 1)) return "FFT width must be even and >= 2";
     s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1;
     if (!factor_radices(s.M, row)) return "FFT width/2 is not 7-smooth";
-    if (!factor_radices(FH, col)) return "FFT height is not 7-smooth";
+    if (!factor_radices(FH, col, true, &col_re)) return "FFT height is not 7-smooth";
+    // the edge kernel holds EDGE_V / RE butterflies of each thread in registers
+    if (col_re && FH / col_re > (EDGE_V / col_re) * ColCfg<T>::NT) col_re = 0;
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
     if (row_lds<T>(s.M) > LDS_BUDGET || !fits(row, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX))
         return "FFT width too large for the LDS-resident row pass";
@@ -388,6 +568,14 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     for (void *fn : fns_col)
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
             return "cannot raise LDS limit for the column pass";
+    if (col_re) {
+        void *fns_e[] = {col_e_fn<T, T, COL_FWD>(col_re), col_e_fn<T, T, COL_INV>(col_re),
+                         col_e_fn<T, T, COL_BCAST>(col_re), col_e_fn<T, T, COL_ADJ>(col_re),
+                         col_e_fn<T, double, COL_ADJ>(col_re)};
+        for (void *fn : fns_e)
+            if (!fn || hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
+                return "cannot raise LDS limit for the column pass";
+    }
     return "";
 }
 
@@ -437,6 +625,20 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in
     hin = std::min(hin, s.FH);
     hout = std::min(hout, s.FH);
     const int irp = rp(in_rows), orp = rp(out_rows), prp = rp(s.FH);
+    if (col_re) {
+        void *fn = mode == COL_FWD     ? col_e_fn<T, T, COL_FWD>(col_re)
+                   : mode == COL_INV   ? col_e_fn<T, T, COL_INV>(col_re)
+                   : mode == COL_BCAST ? col_e_fn<T, T, COL_BCAST>(col_re)
+                                       : col_e_fn<T, TA, COL_ADJ>(col_re);
+        const int nc = s.NC, nexp = batch;
+        const Cx<T> *pin = in, *ppsf = psf;
+        Cx<T> *pout = out;
+        void *args[] = {(void *)&col, (void *)&nc, (void *)&pin, (void *)&in_bstride, (void *)&irp,
+                        (void *)&pout, (void *)&out_bstride, (void *)&orp, (void *)&ppsf, (void *)&psf_bstride,
+                        (void *)&prp, (void *)&hin, (void *)&hout, (void *)&nexp};
+        const dim3 grid(gx, (mode == COL_FWD || mode == COL_INV) ? batch : 1);
+        return hipLaunchKernel(fn, grid, dim3(CNT), args, lds, st);
+    }
     switch (mode) {
         case COL_FWD:
             hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_FWD>), dim3(gx, batch), dim3(CNT), lds, st, col,

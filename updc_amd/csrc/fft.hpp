@@ -30,6 +30,7 @@ template <class T> struct Fft2 {
     FftShape s;
     FftLen row{}, col{};
     Cx<T> *tw_row = nullptr, *tw_col = nullptr, *post = nullptr;  // device tables
+    int col_re = 0;  // edge radix of the column plan (0: LDS-only column kernel)
 
     // Returns "" on success or an error message.
     std::string init(int FW, int FH);
