@@ -85,9 +85,10 @@ def timed_run(c, steps: int, warmup: int):
     return dt, rows, info
 
 
-def stage_profile(c, iters: int = 3) -> dict:
-    """Per-kernel average device times (HIP events on the library stream),
-    from an eager (non-graph) run on a scratch copy of the state."""
+def stage_profile(c, iters: int) -> dict:
+    """Per-kernel average device times over a timed region of `iters` eager
+    (non-graph) iterations: HIP events recorded around every launch on the
+    library's own stream, averaged per stage.  The state is restored after."""
     x = c.estimate(f64=True)
     c.set_graph(False)
     c.iterate(iters)
@@ -161,7 +162,7 @@ def main() -> int:
         return 0
 
     # per-kernel times of the FP32/batched form (eager launches, HIP events)
-    prof = stage_profile(fast)
+    prof = stage_profile(fast, K)
     infos = [fast.exposure_info(k) for k in range(len(ds.geoms))]
     fast.close()
 
@@ -207,6 +208,15 @@ def main() -> int:
     per_stage = {k: dict(ms=round(v[0], 4), GBps=round(sb[k] / (v[0] * 1e-3) / 1e9, 1)) for k, v in kern.items()}
     log(f"per-kernel: {json.dumps(per_stage)}")
 
+    # HBM traffic of the dominant kernel, from the committed FETCH_SIZE /
+    # WRITE_SIZE summary of this workload (rocprofv3 --pmc runs; see
+    # updc_amd/profsum.py for the copy-kernel calibration)
+    traffic = None
+    tpath = ROOT / "profiles" / f"iteration_kernels_config{args.config}_traffic.txt"
+    if tpath.exists():
+        from updc_amd.profsum import read_traffic
+        traffic = read_traffic(tpath).get(dom, {}).get("total")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -230,8 +240,9 @@ def main() -> int:
                    "ref_f32_rel_err_vs_fp64": None if errr is None else {k: errr[k] for k in ("norm_linf", "rel_rms")},
                    "per_kernel": per_stage},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                     "algorithmic_bytes_per_launch": sb[dom], "avg_launch_ms": round(ms, 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                     "algorithmic_bytes_per_launch": sb[dom], "avg_launch_ms": round(ms, 4),
+                     "traffic_source": tpath.name if traffic is not None else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -33,7 +33,11 @@ __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restric
     static_assert((1 << LG) == RB, "RB must be 1, 2, 4 or 8");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int r0 = blockIdx.x * RB, b = blockIdx.y, M = f.L;
+    // XCD-aware: the row blocks sharing a spectrum column's 128-byte lines run
+    // on one XCD, back to back, so its L2 serves them all
+    const int rblk = xcd_tile(blockIdx.x, (rows + RB - 1) / RB);
+    if (rblk < 0) return;
+    const int r0 = rblk * RB, b = blockIdx.y, M = f.L;
     if (MODE == ROW_PLAIN) {
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
@@ -128,7 +132,11 @@ __global__ __launch_bounds__(NT) void k_row_c2r(FftLen f, const Cx<T> *__restric
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);  // (M + 1) * RB slots
-    const int r0 = blockIdx.x * RB, b = blockIdx.y, M = f.L;
+    // XCD-aware: the row blocks sharing a spectrum column's 128-byte lines run
+    // on one XCD, back to back, so its L2 serves them all
+    const int rblk = xcd_tile(blockIdx.x, (rows + RB - 1) / RB);
+    if (rblk < 0) return;
+    const int r0 = rblk * RB, b = blockIdx.y, M = f.L;
     const Cx<T> *y = in + (size_t)b * in_bstride;
     for (int idx = threadIdx.x; idx < ((M + 1) << LG); idx += NT) {
         const int v = idx >> LG, j = idx & (RB - 1);
@@ -593,7 +601,7 @@ hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pit
     if (rows <= 0 || batch <= 0) return hipSuccess;
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     const size_t lds = row_lds<T>(s.M);
-    dim3 grid((rows + RB - 1) / RB, batch);
+    dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
     if (mode == ROW_PLAIN)
         hipLaunchKernelGGL((k_row_r2c<T, NT, VMAX, RB, ROW_PLAIN>), grid, dim3(NT), lds, st, row, post, in, in_bstride,
                            in_pitch, win, ex, wr, W, out, out_bstride, rp(rp_rows), rows);
@@ -609,7 +617,8 @@ hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *
     if (rows <= 0 || batch <= 0) return hipSuccess;
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     const size_t lds = row_lds<T>(s.M);
-    hipLaunchKernelGGL((k_row_c2r<T, NT, VMAX, RB>), dim3((rows + RB - 1) / RB, batch), dim3(NT), lds, st, row, post,
+    const dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
+    hipLaunchKernelGGL((k_row_c2r<T, NT, VMAX, RB>), grid, dim3(NT), lds, st, row, post,
                        in, in_bstride, rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows);
     return hipGetLastError();
 }
