@@ -98,28 +98,24 @@ def stage_profile(c, iters: int) -> dict:
     return t
 
 
-def cpu_baseline(ds, fft_shape) -> dict:
+def cpu_baseline(ds, fft_shape, iters: int = 2) -> dict:
     """The oracle's float32 restatement (numpy pocketfft + C detector loops,
-    single thread) timed on this host for one iteration with N=1 and N=2
-    exposures of the same workload; t(N) = a + b N is extrapolated to N."""
+    single thread) timed on this host: `iters` full iterations over every
+    exposure of the same workload, after its (untimed) set-up."""
     from oracle import oracle as O
-    times = {}
-    for n in (1, 2):
-        ex = [dict(sci=ds.sci[k], wht=ds.wht[k], psf=ds.psfs[k], background=ds.background[k],
-                   pedestal=ds.pedestal[k], linkmap=ds.linkmaps[k]) for k in range(n)]
-        s = O.RefSolver(ds.W, ds.H, ex, "f32", fft_shape=fft_shape)
-        t0 = time.perf_counter()
-        s.iterate(1)
-        times[n] = time.perf_counter() - t0
-        log(f"cpu oracle: N={n} one iteration {times[n]:.2f}s")
-    b = times[2] - times[1]
-    a = times[1] - b
-    N = len(ds.geoms)
-    t = a + b * N
+    ex = [dict(sci=ds.sci[k], wht=ds.wht[k], psf=ds.psfs[k], background=ds.background[k],
+               pedestal=ds.pedestal[k], linkmap=ds.linkmaps[k]) for k in range(len(ds.geoms))]
+    t0 = time.perf_counter()
+    s = O.RefSolver(ds.W, ds.H, ex, "f32", fft_shape=fft_shape)
+    log(f"cpu oracle: set-up {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    s.iterate(iters)
+    t = (time.perf_counter() - t0) / iters
+    log(f"cpu oracle: {iters} iterations, {t:.2f}s per iteration")
     return dict(value=1.0 / t, unit="iters/s", cores=1, kind="port",
-                sample=f"1 iteration of the oracle's f32 path with N=1 and N=2 exposures of this workload "
-                       f"(FFT {fft_shape[0]}x{fft_shape[1]}), t(N)=a+bN extrapolated to N={N}: "
-                       f"t1={times[1]:.2f}s t2={times[2]:.2f}s -> {t:.2f}s/iter; numpy pocketfft in place of FFTW3f")
+                sample=f"{iters} iterations of the oracle's f32 path over all {len(ex)} exposures of this workload "
+                       f"(FFT {fft_shape[0]}x{fft_shape[1]}), after its untimed set-up; {t:.2f}s per iteration; "
+                       f"numpy pocketfft (one thread) in place of FFTW3f")
 
 
 def main() -> int:
