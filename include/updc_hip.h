@@ -50,6 +50,8 @@ extern "C" {
 #define UPDC_FLAG_GRAPH 1u      /* capture each iteration as a hipGraph and replay */
 #define UPDC_FLAG_QUIET 2u      /* no informational stdout                          */
 #define UPDC_FLAG_PER_EXPOSURE 4u /* one launch per exposure and stage (no batching) */
+#define UPDC_FLAG_AUTOTUNE 8u   /* time the candidate FFT shapes on the GPU at prepare
+                                   (when the context was created with a 0 x 0 shape) */
 
 /* ---- LinkMap ABI (updc/cache_abi.py:9-12, updc_core.c:44-54) -------------- */
 #define UPDC_LINKMAP_MAGIC 0x324D4C55u
@@ -109,10 +111,11 @@ typedef struct {
     double ms_total;
 } updc_iter_info;
 
-/* fft_width/fft_height of 0 pick the shape automatically (the smallest even
- * 7-smooth sizes >= target + max PSF - 1, or the autotuner's choice when
- * UPDC_FLAG is set through updc_ctx_set_autotune).  precision is one of
- * UPDC_PREC_*; device is the HIP device ordinal. */
+/* fft_width/fft_height of 0 pick the shape at prepare: the smallest even
+ * 7-smooth width and 7-smooth height >= target + max PSF - 1
+ * (updc_core.c:117-126), or with UPDC_FLAG_AUTOTUNE the fastest candidate
+ * timed on the GPU (updc_core.c:242-291).  precision is one of UPDC_PREC_*;
+ * device is the HIP device ordinal. */
 int updc_ctx_create(updc_ctx **out, int width, int height, int fft_width, int fft_height, int precision,
                     unsigned flags, int device);
 int updc_ctx_destroy(updc_ctx *ctx);
@@ -124,6 +127,29 @@ int updc_ctx_set_flags(updc_ctx *ctx, unsigned flags);
  * 1 = the serial summation order, n > 1 = per-worker partial sums combined in
  * worker order.  Default 1. */
 int updc_ctx_set_preprocess_workers(updc_ctx *ctx, int workers);
+
+/* FFT shape autotuner (replaces select_fft_shape and its cache,
+ * updc_core.c:170-291).  Candidates: the first five admissible sizes per axis
+ * from the minimum (even 7-smooth widths, 7-smooth heights), the diagonal
+ * pairs, then (w2,h0), (w0,h2), (w4,h0), (w0,h4); up to 9 shapes.  Each is
+ * timed as the batched round trip of the iteration's transforms (row R2C,
+ * column forward, column inverse, row C2R over all exposures), with the
+ * edge-stage and the LDS-only column kernel. */
+int updc_fft_shape_candidates(int min_width, int min_height, int *widths, int *heights, int capacity,
+                              int *count);
+/* Cache directory of the autotune result (the reference keeps
+ * fft_autotune_v1_*.txt in the cache root, updc_core.c:181-223); "" or NULL
+ * disables it.  retune != 0 ignores a cached result (updc_core.c:198). */
+int updc_ctx_set_fft_cache(updc_ctx *ctx, const char *dir, int retune);
+/* Column kernel of the 2-D FFT: -1 automatic (edge-stage where the column
+ * length allows), 0 LDS-only, 1 edge-stage where the length allows.  Set
+ * before updc_prepare; no reference counterpart (a tile choice). */
+int updc_ctx_set_column_kernel(updc_ctx *ctx, int kind);
+/* The last autotune: count rows of (width, height, column_edge, ms); ms < 0
+ * marks an unsupported shape.  cached = 1 when the result came from the cache
+ * file (one row). */
+int updc_get_autotune(updc_ctx *ctx, int *widths, int *heights, int *column_edge, double *ms, int capacity,
+                      int *count, int *cached);
 
 /* One exposure (one manifest row, updc_core.c:426-461).  sci/wht: ih*iw raw
  * float32 planes (not yet normalised).  psf: ph*pw float32 raster (stored

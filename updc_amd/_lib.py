@@ -18,6 +18,7 @@ PREC = {"f32": 0, "f64": 1, "mixed": 2}
 FLAG_GRAPH = 1
 FLAG_QUIET = 2
 FLAG_PER_EXPOSURE = 4
+FLAG_AUTOTUNE = 8
 
 
 class UpdcError(RuntimeError):
@@ -40,6 +41,7 @@ class IterInfo(C.Structure):
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int)
 SIGNATURES = {
     "updc_all": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
                            C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]),
@@ -51,6 +53,10 @@ SIGNATURES = {
     "updc_ctx_destroy": (C.c_int, [_P]),
     "updc_ctx_set_flags": (C.c_int, [_P, C.c_uint]),
     "updc_ctx_set_preprocess_workers": (C.c_int, [_P, C.c_int]),
+    "updc_fft_shape_candidates": (C.c_int, [C.c_int, C.c_int, _I, _I, C.c_int, _I]),
+    "updc_ctx_set_fft_cache": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "updc_ctx_set_column_kernel": (C.c_int, [_P, C.c_int]),
+    "updc_get_autotune": (C.c_int, [_P, _I, _I, _I, _D, C.c_int, _I, _I]),
     "updc_add_exposure": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, _P, C.c_size_t, _F, C.c_int, C.c_int, C.c_int,
                                     C.c_double, C.c_double]),
     "updc_add_exposure_vertices": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, _D, _F, C.c_int, C.c_int, C.c_int,

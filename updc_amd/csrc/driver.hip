@@ -149,6 +149,13 @@ struct CtxGuard {
     }
 };
 
+// The cache root is the manifest's directory (updc_core.c:130-138).
+std::string cache_root(const char *manifest) {
+    const std::string m = manifest;
+    const size_t slash = m.rfind('/');
+    return slash == std::string::npos ? std::string(".") : m.substr(0, slash);
+}
+
 int env_int(const char *name, int dflt) {
     const char *v = std::getenv(name);
     return (v && *v) ? std::atoi(v) : dflt;
@@ -157,7 +164,7 @@ int env_int(const char *name, int dflt) {
 void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_t, int iterations,
              const char *prefix, const char *wht_out, int fftw, int expw, int prew, int plan_measure, int autotune,
              int retune, const char *wisdom, int save_every) {
-    (void)fftw; (void)expw; (void)plan_measure; (void)autotune; (void)retune; (void)wisdom;
+    (void)fftw; (void)expw; (void)plan_measure; (void)wisdom;
     if (W < 1 || H < 1 || iw < 1 || ih < 1 || n < 1 || iterations < 1 || fftw < 1 || expw < 1 || prew < 1 ||
         save_every < 0)
         throw DErr(UPDC_ERR_ARG, "invalid dimensions/exposures/iterations/workers/save_every");
@@ -178,9 +185,14 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
     }
     unsigned flags = env_int("UPDC_HIP_GRAPH", 1) ? UPDC_FLAG_GRAPH : 0u;
     if (quiet) flags |= UPDC_FLAG_QUIET;
+    // FFT shape: autotuned on the GPU when asked, with its result cached in
+    // the manifest's directory when the persistent cache is on and no retune
+    // is requested (updc_core.c:130-138, 187-223, 242-291)
+    if (autotune) flags |= UPDC_FLAG_AUTOTUNE;
     CtxGuard g;
     check(updc_ctx_create(&g.c, W, H, 0, 0, prec, flags, env_int("UPDC_HIP_DEVICE", 0)));
     check(updc_ctx_set_preprocess_workers(g.c, prew));
+    if (autotune && persist) check(updc_ctx_set_fft_cache(g.c, cache_root(manifest).c_str(), retune));
 
     double t0 = now();
     double t_links = 0, t_data = 0;

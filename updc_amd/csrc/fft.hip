@@ -548,7 +548,7 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (FW < 2 || FH < 1 || (FW & 1)) return "FFT width must be even and >= 2";
     s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1;
     if (!factor_radices(s.M, row)) return "FFT width/2 is not 7-smooth";
-    if (!factor_radices(FH, col, true, &col_re)) return "FFT height is not 7-smooth";
+    if (!factor_radices(FH, col, allow_edge, &col_re)) return "FFT height is not 7-smooth";
     // the edge kernel holds EDGE_V / RE butterflies of each thread in registers
     if (col_re && FH / col_re > (EDGE_V / col_re) * ColCfg<T>::NT) col_re = 0;
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;

@@ -30,7 +30,8 @@ template <class T> struct Fft2 {
     FftShape s;
     FftLen row{}, col{};
     Cx<T> *tw_row = nullptr, *tw_col = nullptr, *post = nullptr;  // device tables
-    int col_re = 0;  // edge radix of the column plan (0: LDS-only column kernel)
+    int col_re = 0;          // edge radix of the column plan (0: LDS-only column kernel)
+    bool allow_edge = true;  // set before init(): false forces the LDS-only column kernel
 
     // Returns "" on success or an error message.
     std::string init(int FW, int FH);

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <unistd.h>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -231,10 +232,20 @@ struct PipeBase {
     std::vector<int> stage_n;
 };
 
+struct TuneRow {
+    int w, h, edge;  // shape and column kernel (1 edge-stage, 0 LDS-only)
+    double ms;       // batched round trip; < 0 when the shape is unsupported
+};
+
 struct Ctx {
     int W = 0, H = 0, FW = 0, FH = 0, prec = 0, device = 0;
     unsigned flags = 0;
     int workers = 1;
+    int col_edge = -1;           // column kernel: -1 automatic, 0 LDS-only, 1 edge-stage
+    std::string fft_cache_dir;   // autotune cache directory ("" = no cache)
+    bool fft_retune = false;     // ignore a cached autotune result
+    std::vector<TuneRow> tune;   // the last autotune's measurements
+    bool tune_cached = false;
     std::vector<std::unique_ptr<HostExp>> ex;
     hipStream_t st = nullptr;
     bool prepared = false;
@@ -371,6 +382,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     void prepare(Ctx &c) override {
         const int n = N(c);
         const size_t nout = c.nout(), npix = c.npix();
+        fft.allow_edge = c.col_edge != 0;
         std::string e = fft.init(c.FW, c.FH);
         if (!e.empty()) throw Error(UPDC_ERR_ARG, e);
         const FftShape &s = fft.s;
@@ -444,6 +456,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         } else {
             // mixed tier: the denominator is computed entirely in FP64
             Fft2<double> f64;
+            f64.allow_edge = c.col_edge != 0;
             std::string e2 = f64.init(c.FW, c.FH);
             if (!e2.empty()) throw Error(UPDC_ERR_ARG, e2);
             try {
@@ -1013,6 +1026,174 @@ static void need(bool ok, int code, const char *msg) {
     if (!ok) throw Error(code, msg);
 }
 
+// ------------------------------------------------------------------ FFT shape autotuner
+// Candidate shapes, after the reference's selection (updc_core.c:170-179,
+// 242-291): the first five admissible sizes per axis from the minimum (here
+// widths even and 7-smooth, heights 7-smooth, the row pass needing an even
+// width), the diagonal pairs, then (w2,h0), (w0,h2), (w4,h0), (w0,h4).
+static int shape_candidates(int min_w, int min_h, int (*out)[2], int cap) {
+    int wc[5], hc[5];
+    for (int i = 0, v = min_w; i < 5; ++i) v = (wc[i] = next_even_smooth(v)) + 1;
+    for (int i = 0, v = min_h; i < 5; ++i) v = (hc[i] = next_smooth(v)) + 1;
+    int n = 0;
+    auto add = [&](int w, int h) {
+        for (int i = 0; i < n; ++i)
+            if (out[i][0] == w && out[i][1] == h) return;
+        if (n < cap) { out[n][0] = w; out[n][1] = h; ++n; }
+    };
+    for (int i = 0; i < 5; ++i) add(wc[i], hc[i]);
+    add(wc[2], hc[0]);
+    add(wc[0], hc[2]);
+    add(wc[4], hc[0]);
+    add(wc[0], hc[4]);
+    return n;
+}
+
+// Batched round trip of the iteration's transforms for one shape: row R2C
+// of N target-sized images, forward column pass, inverse column pass, row
+// C2R (the reference times one R2C + C2R, updc_core.c:225-240).  Best of
+// three after a warm-up, in ms; < 0 when the shape does not fit the kernels.
+template <class T> static double time_round_trip(Ctx &c, int FW, int FH, bool edge, bool *edge_used) {
+    Fft2<T> f;
+    f.allow_edge = edge;
+    if (!f.init(FW, FH).empty()) {
+        f.release();
+        return -1.0;
+    }
+    *edge_used = f.col_re != 0;
+    const int n = (int)c.ex.size();
+    const size_t nimg = (size_t)c.W * c.H;
+    double best = -1.0;
+    hipEvent_t a = nullptr, b = nullptr;
+    try {
+        DevBuf<T> img;
+        DevBuf<Cx<T>> spec;
+        img.alloc((size_t)n * nimg);
+        spec.alloc((size_t)n * f.spec_elems(FH));
+        HIPCHK(hipMemsetAsync(img.p, 0, (size_t)n * nimg * sizeof(T), c.st));
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        const size_t ss = f.spec_elems(FH);
+        auto trip = [&]() {
+            HIPCHK(f.row_r2c(ROW_PLAIN, img.p, nimg, c.W, c.W, nullptr, nullptr, 0, spec.p, ss, FH, c.H, n, c.st));
+            HIPCHK(f.template colpass<T>(COL_FWD, spec.p, ss, FH, spec.p, ss, FH, nullptr, 0, c.H, FH, n, c.st));
+            HIPCHK(f.template colpass<T>(COL_INV, spec.p, ss, FH, spec.p, ss, FH, nullptr, 0, FH, c.H, n, c.st));
+            HIPCHK(f.row_c2r(spec.p, ss, FH, img.p, nimg, c.W, c.W, (T)1, c.H, n, c.st));
+        };
+        trip();
+        for (int r = 0; r < 3; ++r) {
+            HIPCHK(hipEventRecord(a, c.st));
+            trip();
+            HIPCHK(hipEventRecord(b, c.st));
+            HIPCHK(hipEventSynchronize(b));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, a, b));
+            if (best < 0 || ms < best) best = ms;
+        }
+    } catch (...) {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        f.release();
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    f.release();
+    return best;
+}
+
+static std::string device_signature(int dev) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown-device";
+    return std::string(p.name) + "|" + p.gcnArchName + "|cu" + std::to_string(p.multiProcessorCount) + "|updc_hip 1";
+}
+
+static std::string tune_cache_path(const Ctx &c, int min_w, int min_h) {
+    const char *pn = c.prec == UPDC_PREC_F64 ? "f64" : c.prec == UPDC_PREC_MIXED ? "mixed" : "f32";
+    return c.fft_cache_dir + "/fft_autotune_gpu_v1_" + pn + "_n" + std::to_string(c.ex.size()) + "_" +
+           std::to_string(c.W) + "x" + std::to_string(c.H) + "_" + std::to_string(min_w) + "x" +
+           std::to_string(min_h) + ".txt";
+}
+
+static bool load_tune_cache(Ctx &c, int min_w, int min_h) {
+    if (c.fft_cache_dir.empty() || c.fft_retune) return false;
+    FILE *f = std::fopen(tune_cache_path(c, min_w, min_h).c_str(), "r");
+    if (!f) return false;
+    char sig[512] = {0};
+    int w = 0, h = 0, edge = 0;
+    double ms = 0;
+    const bool ok = std::fscanf(f, "signature=%511[^\n]\nwidth=%d\nheight=%d\ncolumn_edge=%d\nroundtrip_ms=%lf",
+                                sig, &w, &h, &edge, &ms) == 5;
+    std::fclose(f);
+    if (!ok || device_signature(c.device) != sig || w < min_w || h < min_h || (w & 1) || !seven_smooth(w) ||
+        !seven_smooth(h))
+        return false;
+    c.FW = w;
+    c.FH = h;
+    c.col_edge = edge ? 1 : 0;
+    c.tune.assign(1, TuneRow{w, h, c.col_edge, ms});
+    c.tune_cached = true;
+    return true;
+}
+
+static void save_tune_cache(const Ctx &c, int min_w, int min_h, double ms) {
+    if (c.fft_cache_dir.empty()) return;
+    const std::string path = tune_cache_path(c, min_w, min_h);
+    const std::string tmp = path + ".tmp." + std::to_string((long)getpid());
+    FILE *f = std::fopen(tmp.c_str(), "w");
+    if (!f) return;
+    std::fprintf(f, "signature=%s\nwidth=%d\nheight=%d\ncolumn_edge=%d\nroundtrip_ms=%.6f\n",
+                 device_signature(c.device).c_str(), c.FW, c.FH, c.col_edge, ms);
+    if (std::fclose(f) != 0 || std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
+static void autotune_shape(Ctx &c, int min_w, int min_h) {
+    if (load_tune_cache(c, min_w, min_h)) {
+        if (!c.quiet())
+            std::printf("FFT autotune persistent-cache hit: %dx%d, %s column kernel (roundtrip %.3f ms)\n", c.FW,
+                        c.FH, c.col_edge ? "edge-stage" : "LDS-only", c.tune[0].ms);
+        return;
+    }
+    int shapes[16][2];
+    const int ns = shape_candidates(min_w, min_h, shapes, 16);
+    c.tune.clear();
+    c.tune_cached = false;
+    double best = -1;
+    int bw = 0, bh = 0, be = 0;
+    if (!c.quiet()) std::printf("FFT autotune: %d candidate shapes, %zu exposures\n", ns, c.ex.size());
+    for (int i = 0; i < ns; ++i) {
+        for (int edge = 1; edge >= 0; --edge) {
+            bool used = false;
+            const bool f64 = c.prec == UPDC_PREC_F64;
+            const double ms = f64 ? time_round_trip<double>(c, shapes[i][0], shapes[i][1], edge != 0, &used)
+                                  : time_round_trip<float>(c, shapes[i][0], shapes[i][1], edge != 0, &used);
+            if (edge && !used && ms >= 0) {
+                // no edge radix for this height: the LDS-only kernel ran; record it once
+                c.tune.push_back(TuneRow{shapes[i][0], shapes[i][1], 0, ms});
+                if (best < 0 || ms < best) { best = ms; bw = shapes[i][0]; bh = shapes[i][1]; be = 0; }
+                if (!c.quiet()) std::printf("  FFT candidate %dx%d LDS-only roundtrip %.3f ms\n", shapes[i][0], shapes[i][1], ms);
+                break;
+            }
+            c.tune.push_back(TuneRow{shapes[i][0], shapes[i][1], edge, ms});
+            if (!c.quiet()) {
+                if (ms < 0) std::printf("  FFT candidate %dx%d unsupported by the LDS-resident passes\n", shapes[i][0], shapes[i][1]);
+                else std::printf("  FFT candidate %dx%d %s roundtrip %.3f ms\n", shapes[i][0], shapes[i][1],
+                                 edge ? "edge-stage" : "LDS-only", ms);
+            }
+            if (ms < 0) break;
+            if (best < 0 || ms < best) { best = ms; bw = shapes[i][0]; bh = shapes[i][1]; be = edge; }
+        }
+    }
+    if (best < 0) throw Error(UPDC_ERR_ARG, "no FFT shape candidate fits the LDS-resident passes");
+    c.FW = bw;
+    c.FH = bh;
+    c.col_edge = be;
+    if (!c.quiet())
+        std::printf("FFT autotune selected %dx%d, %s column kernel (roundtrip %.3f ms)\n", bw, bh,
+                    be ? "edge-stage" : "LDS-only", best);
+    save_tune_cache(c, min_w, min_h, best);
+}
+
 extern "C" {
 
 const char *updc_last_error(void) { return g_last_error.c_str(); }
@@ -1094,6 +1275,52 @@ int updc_ctx_set_preprocess_workers(updc_ctx *ctx, int workers) {
     });
 }
 
+int updc_fft_shape_candidates(int min_width, int min_height, int *widths, int *heights, int capacity, int *count) {
+    return guarded([&] {
+        need(min_width >= 2 && min_height >= 1 && count && capacity >= 0, UPDC_ERR_ARG, "invalid arguments");
+        int shapes[16][2];
+        const int n = shape_candidates(min_width, min_height, shapes, 16);
+        for (int i = 0; i < n && i < capacity; ++i) {
+            if (widths) widths[i] = shapes[i][0];
+            if (heights) heights[i] = shapes[i][1];
+        }
+        *count = n;
+    });
+}
+
+int updc_ctx_set_fft_cache(updc_ctx *ctx, const char *dir, int retune) {
+    return guarded([&] {
+        need(ctx != nullptr, UPDC_ERR_ARG, "null context");
+        need(!ctx->prepared, UPDC_ERR_STATE, "context already prepared");
+        ctx->fft_cache_dir = dir ? dir : "";
+        ctx->fft_retune = retune != 0;
+    });
+}
+
+int updc_ctx_set_column_kernel(updc_ctx *ctx, int kind) {
+    return guarded([&] {
+        need(ctx != nullptr && kind >= -1 && kind <= 1, UPDC_ERR_ARG, "invalid column kernel kind");
+        need(!ctx->prepared, UPDC_ERR_STATE, "context already prepared");
+        ctx->col_edge = kind;
+    });
+}
+
+int updc_get_autotune(updc_ctx *ctx, int *widths, int *heights, int *column_edge, double *ms, int capacity,
+                      int *count, int *cached) {
+    return guarded([&] {
+        need(ctx != nullptr && count && capacity >= 0, UPDC_ERR_ARG, "invalid arguments");
+        const int n = (int)ctx->tune.size();
+        for (int i = 0; i < n && i < capacity; ++i) {
+            if (widths) widths[i] = ctx->tune[i].w;
+            if (heights) heights[i] = ctx->tune[i].h;
+            if (column_edge) column_edge[i] = ctx->tune[i].edge;
+            if (ms) ms[i] = ctx->tune[i].ms;
+        }
+        *count = n;
+        if (cached) *cached = ctx->tune_cached ? 1 : 0;
+    });
+}
+
 int updc_add_exposure(updc_ctx *ctx, const float *sci, const float *wht, int iw, int ih, const void *linkmap,
                       size_t linkmap_bytes, const float *psf, int pw, int ph, int psf_t, double bg, double ped) {
     return guarded([&] {
@@ -1154,6 +1381,7 @@ int updc_prepare(updc_ctx *ctx) {
             pad_h = std::max(pad_h, e->psf_t ? e->pw : e->ph);
         }
         const int min_w = ctx->W + pad_w - 1, min_h = ctx->H + pad_h - 1;
+        if (ctx->FW == 0 && ctx->FH == 0 && (ctx->flags & UPDC_FLAG_AUTOTUNE)) autotune_shape(*ctx, min_w, min_h);
         if (ctx->FW == 0) ctx->FW = next_even_smooth(min_w);
         if (ctx->FH == 0) ctx->FH = next_smooth(min_h);
         need(ctx->FW >= min_w && ctx->FH >= min_h, UPDC_ERR_ARG, "FFT shape smaller than target + PSF - 1");

@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FreezeRow, IterInfo, check
+from ._lib import FLAG_AUTOTUNE, FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FreezeRow, IterInfo, check
 
 
 def _f32(a) -> np.ndarray:
@@ -51,11 +51,13 @@ class Context:
     or 'mixed'.  graph=True replays each iteration as a captured hipGraph."""
 
     def __init__(self, width: int, height: int, precision: str = "f32", fft_shape: tuple[int, int] | None = None,
-                 graph: bool = False, quiet: bool = True, device: int = 0, preprocess_workers: int = 1):
+                 graph: bool = False, quiet: bool = True, device: int = 0, preprocess_workers: int = 1,
+                 autotune: bool = False, fft_cache_dir: str | None = None, retune: bool = False,
+                 column_kernel: int = -1):
         self.lib = _lib.load()
         self.W, self.H = int(width), int(height)
         self.precision = precision
-        flags = (FLAG_GRAPH if graph else 0) | (FLAG_QUIET if quiet else 0)
+        flags = (FLAG_GRAPH if graph else 0) | (FLAG_QUIET if quiet else 0) | (FLAG_AUTOTUNE if autotune else 0)
         fw, fh = fft_shape if fft_shape else (0, 0)
         h = C.c_void_p()
         check(self.lib.updc_ctx_create(C.byref(h), self.W, self.H, int(fw), int(fh), _lib.PREC[precision], flags,
@@ -68,6 +70,10 @@ class Context:
         self.prepared = False
         if preprocess_workers != 1:
             check(self.lib.updc_ctx_set_preprocess_workers(self.h, int(preprocess_workers)))
+        if fft_cache_dir is not None or retune:
+            check(self.lib.updc_ctx_set_fft_cache(self.h, (fft_cache_dir or "").encode(), int(retune)))
+        if column_kernel != -1:
+            check(self.lib.updc_ctx_set_column_kernel(self.h, int(column_kernel)))
 
     # -- lifetime
     def close(self) -> None:
@@ -91,6 +97,16 @@ class Context:
         """Plain form: one launch per exposure and stage (no batching)."""
         self.flags = (self.flags | FLAG_PER_EXPOSURE) if on else (self.flags & ~FLAG_PER_EXPOSURE)
         check(self.lib.updc_ctx_set_flags(self.h, self.flags))
+
+    def autotune_report(self) -> tuple[list[tuple[int, int, int, float]], bool]:
+        """(width, height, column_edge, ms) per timed candidate of the last
+        autotune (ms < 0: unsupported), and whether it came from the cache."""
+        cap = 32
+        w, h, e = (C.c_int * cap)(), (C.c_int * cap)(), (C.c_int * cap)()
+        ms = (C.c_double * cap)()
+        n, cached = C.c_int(), C.c_int()
+        check(self.lib.updc_get_autotune(self.h, w, h, e, ms, cap, C.byref(n), C.byref(cached)))
+        return [(w[i], h[i], e[i], ms[i]) for i in range(min(n.value, cap))], bool(cached.value)
 
     def set_graph(self, on: bool) -> None:
         self.flags = (self.flags | FLAG_GRAPH) if on else (self.flags & ~FLAG_GRAPH)
