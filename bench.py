@@ -12,7 +12,7 @@ a hipGraph), next to the plain FP64 GPU form and the error of the FP32/batched
 estimate against the FP64 one after the same number of iterations.
 
 Usage: python bench.py [--gpus 1] [--steps 40] [--warmup 2] [--config 2|1]
-                       [--no-cpu] [--no-f64] [--profile]
+                       [--no-cpu] [--no-f64] [--no-autotune] [--profile]
 """
 from __future__ import annotations
 
@@ -57,9 +57,9 @@ def make_dataset(config: int):
     return ds
 
 
-def solve_ctx(ds, precision: str, graph: bool, plain: bool = False):
+def solve_ctx(ds, precision: str, graph: bool, plain: bool = False, autotune: bool = False):
     from updc_amd.solver import Context, Exposure
-    c = Context(ds.W, ds.H, precision, graph=graph)
+    c = Context(ds.W, ds.H, precision, graph=graph, autotune=autotune)
     if plain:
         c.set_per_exposure(True)
     for k in range(len(ds.geoms)):
@@ -68,6 +68,9 @@ def solve_ctx(ds, precision: str, graph: bool, plain: bool = False):
     t0 = time.time()
     c.prepare()
     log(f"{precision}{' graph' if graph else ''}: prepare {time.time() - t0:.2f}s {c.scalars()}")
+    if autotune:
+        rows, _ = c.autotune_report()
+        log("autotune: " + ", ".join(f"{w}x{h}{' edge' if e else ''} {ms:.2f}ms" for w, h, e, ms in rows))
     return c
 
 
@@ -126,6 +129,8 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="FP32/batched form on the default FFT shape instead of the autotuned one")
     ap.add_argument("--profile", action="store_true", help="only the FP32/batched timed run (for rocprofv3)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="first run k_copy_ref over this many GiB (HBM counter scale reference)")
@@ -146,7 +151,7 @@ def main() -> int:
     ds = make_dataset(args.config)
     K, Wm = args.steps, args.warmup
 
-    fast = solve_ctx(ds, "mixed", graph=True)
+    fast = solve_ctx(ds, "mixed", graph=True, autotune=not args.no_autotune)
     sc = fast.scalars()
     FW, FH = sc["fft_width"], sc["fft_height"]
     dt32, rows32, info32 = timed_run(fast, K, Wm)
@@ -225,7 +230,9 @@ def main() -> int:
         "vs_baseline": None, "dtype": "f32 (f64 accumulation/denominator/ratio)", "data": "synthetic",
         "config": {"workload": f"{ds.name}: {len(ds.geoms)}x{ds.geoms[0].iw}^2 exposures -> {ds.W}^2 target, "
                                f"PSF {ds.psfs[0].shape[0]}^2, FFT {FW}x{FH}, {K} iterations",
-                   "form": "FP32/batched (all exposures per launch, hipGraph per iteration)",
+                   "form": "FP32/batched (all exposures per launch, hipGraph per iteration, "
+                           + ("default FFT shape)" if args.no_autotune else "autotuned FFT shape)"),
+                   "fft_shape": [FW, FH],
                    "fp64_plain_iters_per_s": None if its64 is None else round(its64, 3),
                    "fp32_over_fp64_plain": None if its64 is None else round(its32 / its64, 3),
                    "fp64_batched_iters_per_s": None if its64b is None else round(its64b, 3),
