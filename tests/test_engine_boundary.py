@@ -1,0 +1,179 @@
+"""The drop-in boundary: the engine's rules (CPU) and the drop-in executable
+end to end (GPU).
+
+CPU: manifest line format (updc/_engine.py:1300-1310), the 18-argument order
+(_engine.py:1381-1389), the psf_transposed rule (:1384), the output-iteration
+rule (:1409-1415), the FreezeMu CSV row check (:1392-1407) and
+UPDC_PERSIST_CACHE (:1369-1379).
+
+GPU: updc_hip_core --self-test; a 2-iteration solve through files, read back
+with the engine's rules and compared bit for bit with the in-memory context;
+FreezeMu forcing by a background of -100 with the reference's own invariants
+(tests/test_smoke.py:117-136); the error path (exit 1, "UPDC_ALL:" on stderr);
+the persistent LinkMap and autotune caches.
+"""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from updc_amd import engine as E
+
+
+def test_manifest_line_format(tmp_path):
+    r = E.ManifestRow(Path("/d/sci_1.bin"), Path("/d/wht_1.bin"), Path("/d/psf_1.bin"), 9, 7,
+                      Path("/d/vertices_1.bin"), 0.1, 50.0)
+    assert r.line() == "/d/sci_1.bin\t/d/wht_1.bin\t/d/psf_1.bin\t9\t7\t/d/vertices_1.bin\t" \
+                       "0.10000000000000001\t50\n"
+    # %.17g round-trips every double
+    for v in (1e-300, -100.0, 2.0 / 3.0, 51.327478872070316, np.nextafter(1.0, 2.0)):
+        r.background = v
+        assert float(r.line().split("\t")[6]) == v
+    p = E.write_manifest(tmp_path / "manifest.txt", [r, r])
+    assert p.read_text().count("\n") == 2
+
+
+def test_write_exposure_files(tmp_path):
+    sci = np.arange(12, dtype=np.float32).reshape(3, 4)
+    v = np.zeros((4, 5, 2))
+    row = E.write_exposure(tmp_path, 1, sci, sci + 1, np.ones((7, 9), np.float32), v, -1.5, 2.25)
+    assert row.psf_width == 9 and row.psf_height == 7
+    assert np.array_equal(np.fromfile(row.science_raw, "<f4").reshape(3, 4), sci)
+    assert row.vertices.stat().st_size == 4 * 5 * 2 * 8
+    with pytest.raises(ValueError):
+        E.write_exposure(tmp_path, 2, sci, sci, np.ones((3, 3)), np.zeros((3, 4, 2)), 0, 0)
+
+
+def test_core_argv_order():
+    a = E.core_argv("core", "m.txt", 96, 80, 48, 40, 2, True, 20, "it", "w.bin", 4, 2, 3, "measure", False, True,
+                    "wis", 5)
+    assert a == ["core", "m.txt", "96", "80", "48", "40", "2", "1", "20", "it", "w.bin", "4", "2", "3", "1", "0",
+                 "1", "wis", "5"]
+    assert len(a) == 19
+
+
+def test_psf_transposed_rule():
+    assert E.psf_transposed("target", "stored-transposed")
+    assert not E.psf_transposed("target", "native")
+    assert not E.psf_transposed("detector", "stored-transposed")
+
+
+@pytest.mark.parametrize("iters,every,expect", [(2, 1, [1, 2]), (40, 0, [40]), (10, 3, [3, 6, 9, 10]),
+                                                (9, 3, [3, 6, 9]), (2, 5, [2])])
+def test_output_iterations(iters, every, expect):
+    assert E.output_iterations(iters, every) == expect
+
+
+def test_freeze_csv_row_check(tmp_path):
+    hdr = "iteration,exposure,n_new_frozen,n_total_frozen,min_raw_mu,mu_marker,frozen_ratio\n"
+    p = tmp_path / "f.csv"
+    p.write_text(hdr + "1,1,0,0,1.5,1e-12,1\n1,2,3,3,-0.25,1e-12,1\n2,1,0,0,1,1e-12,1\n")
+    with pytest.raises(ValueError):
+        E.read_freeze_csv(p, 2, 2)
+    p.write_text(hdr + "1,1,0,0,1.5,1e-12,1\n1,2,3,3,-0.25,1e-12,1\n2,1,0,0,1,1e-12,1\n2,2,1,4,-1,1e-12,1\n")
+    rows = E.read_freeze_csv(p, 2, 2)
+    assert rows[1]["n_new_frozen"] == 3 and rows[3]["n_total_frozen"] == 4 and rows[1]["min_raw_mu"] == -0.25
+
+
+def test_solver_env():
+    assert E.solver_env(True, {})["UPDC_PERSIST_CACHE"] == "1"
+    assert E.solver_env(False, {"X": "y"}) == {"X": "y", "UPDC_PERSIST_CACHE": "0"}
+
+
+# ----------------------------------------------------------------------- GPU
+def _files(ds, work):
+    rows = [E.write_exposure(work, k + 1, ds.sci[k], ds.wht[k], ds.psfs[k], ds.vertices(k), ds.background[k],
+                             ds.pedestal[k]) for k in range(len(ds.geoms))]
+    return E.write_manifest(work / "manifest.txt", rows), rows
+
+
+def _argv(ds, work, manifest, iterations, save_every, prefix="iter", autotune=False):
+    from updc_amd._lib import EXE_PATH
+    g = ds.geoms[0]
+    return E.core_argv(EXE_PATH, manifest, ds.W, ds.H, g.iw, g.ih, len(ds.geoms), False, iterations,
+                       work / prefix, work / f"{prefix}_wht.bin", fft_autotune=autotune, save_every=save_every)
+
+
+@pytest.mark.gpu
+def test_dropin_executable_end_to_end(small_ds, tmp_path):
+    from updc_amd._lib import EXE_PATH
+    from updc_amd.solver import Context, Exposure
+    ds = small_ds
+    print("case: drop-in self-test", flush=True)
+    st = subprocess.run([str(EXE_PATH), "--self-test"], capture_output=True, text=True, timeout=60)
+    assert st.returncode == 0 and "UPDC C core self-test: OK" in st.stdout
+
+    print("case: drop-in 2 iterations, save_every 1", flush=True)
+    manifest, rows = _files(ds, tmp_path)
+    env = E.solver_env(False)
+    env["UPDC_HIP_QUIET"] = "1"
+    res = E.run_core(_argv(ds, tmp_path, manifest, 2, 1), env=env, timeout=120)
+    assert sorted(res.iterations) == [1, 2]
+    assert len(res.freeze_rows) == 2 * len(ds.geoms)
+    assert np.isfinite(res.iterations[2]).all() and np.isfinite(res.wht).all()
+    assert "total_solver" in res.timing
+
+    # the same solve in memory (F32 tier, LinkMaps built on the GPU from the vertices)
+    c = Context(ds.W, ds.H, "f32", graph=True)
+    for k in range(len(ds.geoms)):
+        c.add_exposure(Exposure(ds.sci[k], ds.wht[k], ds.psfs[k], ds.background[k], ds.pedestal[k],
+                                vertices=ds.vertices(k)))
+    c.prepare()
+    fr, _ = c.iterate(2)
+    assert np.array_equal(c.estimate(), res.iterations[2])
+    assert np.array_equal(c.wht(), res.wht)
+    assert [(r[2], r[3]) for r in fr] == [(r["n_new_frozen"], r["n_total_frozen"]) for r in res.freeze_rows]
+    c.close()
+    # LinkMaps were written next to the vertices, in the ULM2 format
+    assert all(Path(str(r.vertices) + ".links").exists() for r in rows)
+
+
+@pytest.mark.gpu
+def test_dropin_freeze_forcing(small_ds, tmp_path):
+    """Background -100 forces non-positive forward means (test_smoke.py:117-136)."""
+    ds = small_ds
+    print("case: drop-in FreezeMu forcing", flush=True)
+    manifest, _ = _files(ds, tmp_path)
+    lines = []
+    for line in manifest.read_text().splitlines():
+        f = line.split("\t")
+        f[6] = "-100"
+        lines.append("\t".join(f))
+    manifest.write_text("\n".join(lines) + "\n")
+    res = E.run_core(_argv(ds, tmp_path, manifest, 2, 1, prefix="forced"), env=E.solver_env(False), timeout=120)
+    rows = res.freeze_rows
+    assert any(r["n_new_frozen"] > 0 for r in rows)
+    for k in range(1, len(ds.geoms) + 1):
+        tot = [r["n_total_frozen"] for r in rows if r["exposure"] == k]
+        new = [r["n_new_frozen"] for r in rows if r["exposure"] == k]
+        assert tot[0] == new[0]
+        assert tot[1] == tot[0] + new[1]
+
+
+@pytest.mark.gpu
+def test_dropin_error_path(small_ds, tmp_path):
+    ds = small_ds
+    print("case: drop-in error path", flush=True)
+    manifest, rows = _files(ds, tmp_path)
+    rows[0].science_raw = tmp_path / "missing.bin"
+    E.write_manifest(manifest, rows)
+    with pytest.raises(subprocess.CalledProcessError) as ei:
+        E.run_core(_argv(ds, tmp_path, manifest, 1, 0), env=E.solver_env(False), timeout=120)
+    assert ei.value.returncode == 1
+    assert "UPDC_ALL:" in ei.value.stderr
+
+
+@pytest.mark.gpu
+def test_dropin_persistent_caches(small_ds, tmp_path):
+    ds = small_ds
+    print("case: drop-in persistent caches", flush=True)
+    manifest, _ = _files(ds, tmp_path)
+    env = E.solver_env(True)
+    a = E.run_core(_argv(ds, tmp_path, manifest, 2, 0, prefix="a", autotune=True), env=env, timeout=120)
+    assert list(tmp_path.glob("fft_autotune_gpu_v1_*.txt"))
+    out = subprocess.run(_argv(ds, tmp_path, manifest, 2, 0, prefix="b", autotune=True), env=env,
+                         capture_output=True, text=True, timeout=120, check=True).stdout
+    assert "persistent cache hit" in out and "FFT autotune persistent-cache hit" in out
+    b = E.run_core(_argv(ds, tmp_path, manifest, 2, 0, prefix="b", autotune=True), env=env, timeout=120)
+    assert np.array_equal(a.iterations[2], b.iterations[2])
