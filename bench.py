@@ -3,7 +3,9 @@
 
 Workload (default): 8 exposures of 2048^2 -> 4096^2 target, Airy+wing PSFs of
 1431^2, FFT 5600^2, 40 iterations (synthetic, seeded; see updc_amd/synth.py).
-A "step" is one reconstruction iteration over all exposures.
+A "step" is one reconstruction iteration over all exposures.  --config 3 runs
+the FreezeMu stress case (errors at iterations 10, 20, 40); --config 4 the
+uneven batch of 24 exposures against per-exposure launches.
 
 Reported line: iterations/s of the FP32/batched form (FP32 FFTs, PSF and S_k
 products, FP64 spectrum accumulation over exposures, denominator, raw_mu test
@@ -11,7 +13,7 @@ and ratio; all exposures batched in every launch; each iteration replayed as
 a hipGraph), next to the plain FP64 GPU form and the error of the FP32/batched
 estimate against the FP64 one after the same number of iterations.
 
-Usage: python bench.py [--gpus 1] [--steps 40] [--warmup 2] [--config 2|1]
+Usage: python bench.py [--gpus 1] [--steps 40] [--warmup 2] [--config 2|1|3|4]
                        [--no-cpu] [--no-f64] [--no-autotune] [--profile]
 """
 from __future__ import annotations
@@ -40,7 +42,7 @@ def make_dataset(config: int):
     from updc_amd import synth
     from updc_amd.solver import Context, Exposure
     t0 = time.time()
-    ds = synth.config2() if config == 2 else synth.config1()
+    ds = {1: synth.config1, 2: synth.config2, 3: synth.config3, 4: synth.config4}[config]()
     log(f"synthetic {ds.name}: truth/PSFs/geometry in {time.time() - t0:.1f}s")
     t0 = time.time()
     n = len(ds.geoms)
@@ -86,6 +88,128 @@ def timed_run(c, steps: int, warmup: int):
     rows, info = c.iterate(steps)    # synchronises before returning
     dt = time.perf_counter() - t0
     return dt, rows, info
+
+
+def timed_chunks(c, chunks: list[int], warmup: int):
+    """Like timed_run, in chunks: the state after each chunk is read back
+    outside the timed region.  Returns (seconds, {iteration: x}, rows)."""
+    x0, _ = c.initial()
+    if warmup:
+        c.iterate(warmup)
+    c.set_estimate(x0)
+    c.reset_frozen()
+    dt, done, snaps, rows = 0.0, 0, {}, []
+    for n in chunks:
+        t0 = time.perf_counter()
+        r, _ = c.iterate(n)
+        dt += time.perf_counter() - t0
+        done += n
+        rows += r
+        snaps[done] = c.estimate(f64=True)
+    return dt, snaps, rows
+
+
+def frozen_totals(rows, nexp: int) -> int:
+    """Frozen pixels over all exposures after the last iteration in rows."""
+    return int(sum(r[3] for r in rows[-nexp:]))
+
+
+def run_stress(args, ds, metrics) -> dict:
+    """Config 3: FP32/batched against plain FP64 under the FreezeMu stress
+    (backgrounds that start ~20% of the pixels of 6 of the 8 exposures at
+    raw_mu <= 0), errors at iterations 10, 20 and 40."""
+    from updc_amd import synth
+    c = solve_ctx(ds, "f64", graph=False)
+    x0, _ = c.initial()
+    mx0 = c.model(x0)
+    c.close()
+    ds.background = synth.stress_backgrounds(mx0, ds.wht, ds.pedestal)
+    log(f"stress backgrounds: {[round(b, 3) for b in ds.background]}")
+    K = args.steps
+    chunks = [10, 10, K - 20] if K > 20 else [K]
+    fast = solve_ctx(ds, "mixed", graph=True, autotune=not args.no_autotune)
+    sc = fast.scalars()
+    dt32, s32, rows32 = timed_chunks(fast, chunks, args.warmup)
+    fast.close()
+    its32 = K / dt32
+    log(f"FP32/batched (mixed, graph): {K} iterations -> {its32:.1f} it/s; frozen {frozen_totals(rows32, len(ds.geoms))}")
+    slow = solve_ctx(ds, "f64", graph=False, plain=True)
+    dt64, s64, rows64 = timed_chunks(slow, chunks, args.warmup)
+    slow.close()
+    its64 = K / dt64
+    log(f"plain FP64: {its64:.1f} it/s; frozen {frozen_totals(rows64, len(ds.geoms))}")
+    errs = {}
+    for it in sorted(s32):
+        st = metrics.stats(s64[it], s32[it])
+        errs[str(it)] = {k: st[k] for k in ("norm_linf", "rel_rms", "max_rel_masked", "max_abs", "rms")}
+        log(f"iteration {it}: FP32/batched vs FP64 {errs[str(it)]}")
+    mism = sum(1 for a, b in zip(rows32, rows64) if a[2:4] != b[2:4])
+    nexp = len(ds.geoms)
+    first = [int(r[3]) for r in rows64[:nexp]]
+    log(f"frozen after iteration 1 per exposure (FP64): {first} of {[int(g.iw * g.ih) for g in ds.geoms]}")
+    return {
+        "metric": METRIC, "value": round(its32, 3), "unit": "iters/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": round(dt32 * 1e3 / K, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 accumulation/denominator/ratio)",
+        "data": "synthetic",
+        "config": {"workload": f"config3 (FreezeMu stress): config-2 geometry and PSFs, sky 0.01, "
+                               f"b_k = -4 (P20(H_k x0) + ped_k) on 6 of 8 exposures, FFT {sc['fft_width']}x{sc['fft_height']}, {K} iterations",
+                   "fp64_plain_iters_per_s": round(its64, 3), "fp32_over_fp64_plain": round(its32 / its64, 3),
+                   "rel_err_vs_fp64_at": errs,
+                   "frozen_fp32": frozen_totals(rows32, len(ds.geoms)),
+                   "frozen_fp64": frozen_totals(rows64, len(ds.geoms)),
+                   "frozen_after_iteration_1_fp64": first,
+                   "freeze_row_mismatches": mism, "backgrounds": ds.background},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def run_uneven(args, ds, metrics) -> dict:
+    """Config 4: 24 exposures of three sizes in one launch per stage (graph,
+    autotuned shape) against per-exposure launches, and the error vs FP64."""
+    K = args.steps
+    fast = solve_ctx(ds, "mixed", graph=True, autotune=not args.no_autotune)
+    sc = fast.scalars()
+    dt32, _, _ = timed_run(fast, K, args.warmup)
+    x32 = fast.estimate(f64=True)
+    fast.close()
+    its32 = K / dt32
+    log(f"FP32/batched (mixed, graph, autotuned): {its32:.2f} it/s")
+    plain = solve_ctx(ds, "mixed", graph=False, plain=True)
+    dtp, _, _ = timed_run(plain, K, args.warmup)
+    xp = plain.estimate(f64=True)
+    plain.close()
+    itsp = K / dtp
+    log(f"per-exposure launches (mixed, eager, default shape): {itsp:.2f} it/s")
+    errs = None
+    its64 = None
+    if not args.no_f64:
+        b64 = solve_ctx(ds, "f64", graph=True)
+        dt64, _, _ = timed_run(b64, K, args.warmup)
+        x64 = b64.estimate(f64=True)
+        b64.close()
+        its64 = K / dt64
+        e = metrics.stats(x64, x32)
+        ep = metrics.stats(x64, xp)
+        errs = {"batched": {k: e[k] for k in ("norm_linf", "rel_rms", "max_rel_masked")},
+                "per_exposure": {k: ep[k] for k in ("norm_linf", "rel_rms", "max_rel_masked")}}
+        log(f"batched FP64: {its64:.2f} it/s; errors vs FP64 {errs}")
+    sizes = {}
+    for g in ds.geoms:
+        sizes[f"{g.iw}x{g.ih}"] = sizes.get(f"{g.iw}x{g.ih}", 0) + 1
+    return {
+        "metric": METRIC, "value": round(its32, 3), "unit": "iters/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": round(dt32 * 1e3 / K, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 accumulation/denominator/ratio)",
+        "data": "synthetic",
+        "config": {"workload": f"config4 (uneven batching): {len(ds.geoms)} exposures {sizes} -> {ds.W}^2 target, "
+                               f"PSFs 1431^2/511^2, FFT {sc['fft_width']}x{sc['fft_height']}, {K} iterations",
+                   "per_exposure_launch_iters_per_s": round(itsp, 3),
+                   "batched_over_per_exposure": round(its32 / itsp, 3),
+                   "fp64_batched_iters_per_s": None if its64 is None else round(its64, 3),
+                   "rel_err_vs_fp64": errs},
+        "roofline": None, "cpu_baseline": None,
+    }
 
 
 def stage_profile(c, iters: int) -> dict:
@@ -150,6 +274,10 @@ def main() -> int:
             f"-> {2 * nbytes / (ms.value * 1e-3) / 1e9:.1f} GB/s")
     ds = make_dataset(args.config)
     K, Wm = args.steps, args.warmup
+    if args.config in (3, 4):
+        line = (run_stress if args.config == 3 else run_uneven)(args, ds, metrics)
+        print(json.dumps(line), flush=True)
+        return 0
 
     fast = solve_ctx(ds, "mixed", graph=True, autotune=not args.no_autotune)
     sc = fast.scalars()

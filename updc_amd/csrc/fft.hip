@@ -554,7 +554,9 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
     if (row_lds<T>(s.M) > LDS_BUDGET || !fits(row, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX))
         return "FFT width too large for the LDS-resident row pass";
-    if ((size_t)FH * sizeof(Cx<T>) > LDS_BUDGET || FH > CNT * CXMAX || !fits(col, 1, CNT, CVMAX))
+    // the LDS-only column kernel keeps XMAX values per thread in registers; the
+    // edge-stage kernel covers the column with its edge registers instead
+    if ((size_t)FH * sizeof(Cx<T>) > LDS_BUDGET || (col_re == 0 && FH > CNT * CXMAX) || !fits(col, 1, CNT, CVMAX))
         return "FFT height too large for the LDS-resident column pass";
     std::string err;
     tw_row = upload_stage_twiddles<T>(row, err);

@@ -455,8 +455,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             denominator<TF>(c, fft, psf_spec.p, d_ex.p, w.p, (TF *)D.p);
         } else {
             // mixed tier: the denominator is computed entirely in FP64
-            Fft2<double> f64;
-            f64.allow_edge = c.col_edge != 0;
+            Fft2<double> f64;  // edge-stage columns where possible: the FP64 LDS-only kernel stops at FH 6144
             std::string e2 = f64.init(c.FW, c.FH);
             if (!e2.empty()) throw Error(UPDC_ERR_ARG, e2);
             try {

@@ -173,6 +173,77 @@ def config2(n_exp: int = 8, det: int = 2048, target: int = 4096, psf_size: int =
     return Dataset(target, target, geoms, psfs, truth, [0.0] * n_exp, 10.0, 0.01, None, 40, "config2")
 
 
+def config3() -> Dataset:
+    """Low-sky stress case of the config-2 workload: the same geometry and
+    PSFs, sky 0.01.  The background stress (per-exposure b_k that start about
+    20% of the pixels of 6 of the 8 exposures at raw_mu <= 0) is applied
+    after the initial estimate exists (see stress_backgrounds)."""
+    ds = config2()
+    ds.truth = truth_image(ds.W, ds.H, 5000, 0.01, rng_for(3, "truth"), sersic_fraction=0.5)
+    ds.name = "config3"
+    return ds
+
+
+def stress_backgrounds(model_x0: list[np.ndarray], wht: list[np.ndarray], pedestal: list[float],
+                       q: float = 20.0, stressed_fraction: float = 0.75) -> list[float]:
+    """Backgrounds that start about q% of the pixels of most exposures at
+    raw_mu <= 0.  model_x0[k] is H_k x0 for the initial estimate x0 computed
+    with zero backgrounds.  The initial estimate subtracts the (weighted) mean
+    background, so raw_mu_k(x0) = H_k x0 + ped_k + (b_k - mean b): only the
+    differences between exposures move it.  The first `stressed_fraction` of
+    the exposures get b_k = -a_k / (1 - f) with a_k = P_q(H_k x0) + ped_k,
+    the rest 0, so a stressed exposure starts near H_k x0 - P_q(H_k x0)."""
+    n = len(model_x0)
+    ns = int(round(stressed_fraction * n))
+    f = ns / n
+    out = []
+    for k, (m, w, p) in enumerate(zip(model_x0, wht, pedestal)):
+        a = float(np.percentile(m[w > 0], q)) + float(p)
+        out.append(-a / (1.0 - f) if k < ns and f < 1.0 else 0.0)
+    return out
+
+
+def gaussian_mixture_psf(size: int, rng: np.random.Generator, ncomp: int = 3,
+                         fwhm_range=(1.8, 3.0)) -> np.ndarray:
+    """Normalised sum of `ncomp` Gaussians, FWHM U[fwhm_range], weights
+    U[0.2, 1], centroid offsets N(0, 0.3) px."""
+    c = (size - 1) / 2.0
+    y, x = np.mgrid[0:size, 0:size].astype(np.float64)
+    p = np.zeros((size, size))
+    for _ in range(ncomp):
+        s = float(rng.uniform(*fwhm_range)) / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+        dx, dy = rng.normal(0.0, 0.3, 2)
+        p += float(rng.uniform(0.2, 1.0)) * np.exp(-((x - c - dx) ** 2 + (y - c - dy) ** 2) / (2 * s * s))
+    return (p / p.sum()).astype(np.float32)
+
+
+def config4(target: int = 6144, nsrc: int = 11250) -> Dataset:
+    """Uneven batching: N=24 exposures in one launch -- 12 of 2048^2, 8
+    subarrays of 1024 (wide) x 2048 (high) and 4 of 640^2 -- at 2x scale on a
+    6144^2 target, dithers U[0,40] px, rotations U[-2,2] deg; each with its
+    own 3-component Gaussian-mixture PSF (FWHM U[1.8,3] px) of 1431^2 (even
+    exposures) or 511^2 (odd).  Sources scale with the area of config 2."""
+    g = rng_for(4, "geometry")
+    sizes = [(2048, 2048)] * 12 + [(1024, 2048)] * 8 + [(640, 640)] * 4
+    geoms = []
+    for i, (iw, ih) in enumerate(sizes):
+        fw, fh = 2 * iw, 2 * ih  # footprint in target pixels
+        # full detectors tile the target in a 2x2 pattern; subarrays land anywhere inside it
+        if i < 12:
+            cx = fw / 2 + (target - fw) * (i % 2)
+            cy = fh / 2 + (target - fh) * ((i // 2) % 2)
+        else:
+            cx = float(g.uniform(fw / 2, target - fw / 2))
+            cy = float(g.uniform(fh / 2, target - fh / 2))
+        th = float(g.uniform(-2.0, 2.0))
+        d = (float(g.uniform(0, 40)), float(g.uniform(0, 40)))
+        geoms.append(Geometry(iw, ih, 2.0, th, d, None, (cx - 0.5 - 40.0, cy - 0.5 - 40.0)))
+    pr = rng_for(4, "psf")
+    psfs = [gaussian_mixture_psf(1431 if i % 2 == 0 else 511, pr) for i in range(len(sizes))]
+    truth = truth_image(target, target, nsrc, 0.3, rng_for(4, "truth"), sersic_fraction=0.5)
+    return Dataset(target, target, geoms, psfs, truth, [0.0] * len(sizes), 10.0, 0.01, None, 40, "config4")
+
+
 def small_config(n_exp=2, det=48, target=96, psf=9, seed=7, rotation=3.0, dither=3.0, distort=0.0,
                  nsrc=20, sky=0.5, mask_fraction=0.02, read_noise=2.0) -> Dataset:
     """Tiny dataset for oracle-sized parity tests."""
