@@ -214,6 +214,33 @@ int updc_stage_scatter(updc_ctx *ctx, int k, const double *v, double *out);
 /* Zero every FreezeMu mask. */
 int updc_reset_frozen(updc_ctx *ctx);
 
+/* ---- device-pointer operators (the autograd wrapper) ------------------------
+ * On a prepared context, with DEVICE pointers of the context's element type
+ * (float for UPDC_PREC_F32 and UPDC_PREC_MIXED, double for UPDC_PREC_F64),
+ * enqueued on `stream` (a hipStream_t; NULL is HIP's null stream, as
+ * everywhere in HIP -- torch's default stream) without synchronising.  Spectra are column-major half spectra: complex
+ * element (row r, column v) at v * pitch + r, spectrum_columns = FW/2 + 1
+ * columns of spectrum_pitch >= FH rows; rows >= FH are padding and never
+ * read.  Calls on one context share a workspace: issue them on one stream.
+ * No reference counterpart as an API: they expose the iteration's own stages
+ * (ws_forward_target / ws_inverse_crop, updc_core.c:728-776; the LinkMap
+ * gather and scatter, updc_core.c:1154-1176). */
+int updc_op_layout(updc_ctx *ctx, int *fft_width, int *fft_height, int *spectrum_columns, int *spectrum_pitch,
+                   int *element_bytes);
+/* spectrum = R2C of the rows x cols image (row pitch `pitch` elements), zero
+ * padded to FH x FW; unnormalised. */
+int updc_op_r2c(updc_ctx *ctx, const void *image, int rows, int cols, int pitch, void *spectrum, void *stream);
+/* image = scale * C2R(spectrum), cropped to rows x cols (row pitch `pitch`). */
+int updc_op_c2r(updc_ctx *ctx, const void *spectrum, void *image, int rows, int cols, int pitch, double scale,
+                void *stream);
+/* out[p] = sum_z frac_z image[t_z] for the nin_k pixels of exposure k (S_k,
+ * the fmaf chain of updc_core.c:1154-1158); image is H x W. */
+int updc_op_gather(updc_ctx *ctx, int k, const void *image, void *out, void *stream);
+/* image[t] = sum_p frac v[p] over the pixels with w > 0 (S_k^T restricted to
+ * the weighted pixels, the transposed gather of updc_core.c:1171-1176);
+ * v has nin_k elements, image is H x W and fully written. */
+int updc_op_scatter(updc_ctx *ctx, int k, const void *v, void *image, void *stream);
+
 /* 2-D FFT self-check (the GPU transforms on their own): image FH x FW real ->
  * half spectrum FH x (FW/2+1) interleaved complex, and back (unnormalised). */
 int updc_fft_r2c_f64(updc_ctx *ctx, const double *image, double *spectrum);

@@ -57,6 +57,11 @@ SIGNATURES = {
     "updc_ctx_set_fft_cache": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "updc_ctx_set_column_kernel": (C.c_int, [_P, C.c_int]),
     "updc_get_autotune": (C.c_int, [_P, _I, _I, _I, _D, C.c_int, _I, _I]),
+    "updc_op_layout": (C.c_int, [_P, _I, _I, _I, _I, _I]),
+    "updc_op_r2c": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "updc_op_c2r": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, _P]),
+    "updc_op_gather": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "updc_op_scatter": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "updc_add_exposure": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, _P, C.c_size_t, _F, C.c_int, C.c_int, C.c_int,
                                     C.c_double, C.c_double]),
     "updc_add_exposure_vertices": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, _D, _F, C.c_int, C.c_int, C.c_int,

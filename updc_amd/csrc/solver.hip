@@ -226,6 +226,12 @@ struct PipeBase {
     virtual void fft_c2r(Ctx &c, const double *spec, double *img) = 0;
     virtual void get_frozen(Ctx &c, int k, uint8_t *out) = 0;
     virtual void reset_frozen(Ctx &c) = 0;
+    // device-pointer operators (element type TF), on the caller's stream
+    virtual void op_r2c(Ctx &c, const void *img, int rows, int cols, int pitch, void *spec, hipStream_t st) = 0;
+    virtual void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double scale,
+                        hipStream_t st) = 0;
+    virtual void op_gather(Ctx &c, int k, const void *img, void *out, hipStream_t st) = 0;
+    virtual void op_scatter(Ctx &c, int k, const void *v, void *img, hipStream_t st) = 0;
     double max_sens = 0;
     std::vector<std::string> stage_names;
     std::vector<double> stage_ms;
@@ -871,6 +877,36 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, c.npix(), c.st));
         HIPCHK(hipStreamSynchronize(c.st));
     }
+
+    // ---- device-pointer operators.  Spectra are column-major, NC columns of
+    // pitch rp(FH) (element (r, v) at v * rp(FH) + r), as everywhere here.
+    DevBuf<Cx<TF>> op_ws;  // column-inverse workspace of op_c2r
+    void op_r2c(Ctx &c, const void *img, int rows, int cols, int pitch, void *spec, hipStream_t st) override {
+        (void)c;
+        const FftShape &s = fft.s;
+        Cx<TF> *sp = (Cx<TF> *)spec;
+        HIPCHK(fft.row_r2c(ROW_PLAIN, (const TF *)img, 0, pitch, cols, nullptr, nullptr, 0, sp, 0, s.FH, rows, 1, st));
+        HIPCHK(fft.template colpass<TA>(COL_FWD, sp, 0, s.FH, sp, 0, s.FH, nullptr, 0, rows, s.FH, 1, st));
+    }
+    void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double sc,
+                hipStream_t st) override {
+        (void)c;
+        const FftShape &s = fft.s;
+        if (!op_ws.p) op_ws.alloc(fft.spec_elems(s.FH));
+        HIPCHK(fft.template colpass<TA>(COL_INV, (const Cx<TF> *)spec, 0, s.FH, op_ws.p, 0, s.FH, nullptr, 0, s.FH,
+                                        rows, 1, st));
+        HIPCHK(fft.row_c2r(op_ws.p, 0, s.FH, (TF *)img, 0, pitch, cols, (TF)sc, rows, 1, st));
+    }
+    void op_gather(Ctx &c, int k, const void *img, void *out, hipStream_t st) override {
+        // the kernels index per-pixel arrays at pix_k + p: shift the caller's
+        // nin_k buffer by -pix_k (only [pix_k, pix_k + nin_k) is touched)
+        const HostExp &e = *c.ex[k];
+        HIPCHK(predict<TF>(d_ex.p + k, 1, (uint32_t)e.nin, (const TF *)img, 0, (TF *)out - e.pix, st));
+    }
+    void op_scatter(Ctx &c, int k, const void *v, void *img, hipStream_t st) override {
+        const HostExp &e = *c.ex[k];
+        HIPCHK(csc_apply<TF>(d_ex.p + k, (const TF *)v - e.pix, c.nout(), (TF *)img, st));
+    }
 };
 
 // ------------------------------------------------------------------ exposures
@@ -1391,6 +1427,9 @@ int updc_prepare(updc_ctx *ctx) {
         else ctx->pipe = std::make_unique<Pipe<float, double>>();
         try {
             ctx->pipe->prepare(*ctx);
+            // the context's stream does not order against other streams: finish the
+            // set-up before operators run on the caller's stream
+            HIPCHK(hipStreamSynchronize(ctx->st));
         } catch (...) {
             ctx->pipe.reset();
             throw;
@@ -1493,6 +1532,54 @@ int updc_stage_scatter(updc_ctx *ctx, int k, const double *v, double *out) {
         ctx->pipe->stage_scatter(*ctx, k, v, out);
     });
 }
+// ---- device-pointer operators
+int updc_op_layout(updc_ctx *ctx, int *fft_width, int *fft_height, int *spectrum_columns, int *spectrum_pitch,
+                   int *element_bytes) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        if (fft_width) *fft_width = ctx->FW;
+        if (fft_height) *fft_height = ctx->FH;
+        if (spectrum_columns) *spectrum_columns = ctx->FW / 2 + 1;
+        if (spectrum_pitch) *spectrum_pitch = rp(ctx->FH);
+        if (element_bytes) *element_bytes = ctx->prec == UPDC_PREC_F64 ? 8 : 4;
+    });
+}
+int updc_op_r2c(updc_ctx *ctx, const void *image, int rows, int cols, int pitch, void *spectrum, void *stream) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(image && spectrum && rows >= 1 && rows <= ctx->FH && cols >= 1 && cols <= ctx->FW && pitch >= cols,
+             UPDC_ERR_ARG, "invalid operator arguments");
+        HIPCHK(hipSetDevice(ctx->device));
+        ctx->pipe->op_r2c(*ctx, image, rows, cols, pitch, spectrum, (hipStream_t)stream);
+    });
+}
+int updc_op_c2r(updc_ctx *ctx, const void *spectrum, void *image, int rows, int cols, int pitch, double scale,
+                void *stream) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(image && spectrum && rows >= 1 && rows <= ctx->FH && cols >= 1 && cols <= ctx->FW && pitch >= cols,
+             UPDC_ERR_ARG, "invalid operator arguments");
+        HIPCHK(hipSetDevice(ctx->device));
+        ctx->pipe->op_c2r(*ctx, spectrum, image, rows, cols, pitch, scale, (hipStream_t)stream);
+    });
+}
+int updc_op_gather(updc_ctx *ctx, int k, const void *image, void *out, void *stream) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(image && out && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid operator arguments");
+        HIPCHK(hipSetDevice(ctx->device));
+        ctx->pipe->op_gather(*ctx, k, image, out, (hipStream_t)stream);
+    });
+}
+int updc_op_scatter(updc_ctx *ctx, int k, const void *v, void *image, void *stream) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(v && image && k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid operator arguments");
+        HIPCHK(hipSetDevice(ctx->device));
+        ctx->pipe->op_scatter(*ctx, k, v, image, (hipStream_t)stream);
+    });
+}
+
 int updc_fft_r2c_f64(updc_ctx *ctx, const double *image, double *spectrum) {
     return guarded([&] { NEED_PREPARED(ctx); need(image && spectrum, UPDC_ERR_ARG, "null"); ctx->pipe->fft_r2c(*ctx, image, spectrum); });
 }

@@ -57,6 +57,7 @@ class Context:
         self.lib = _lib.load()
         self.W, self.H = int(width), int(height)
         self.precision = precision
+        self.device = int(device)
         flags = (FLAG_GRAPH if graph else 0) | (FLAG_QUIET if quiet else 0) | (FLAG_AUTOTUNE if autotune else 0)
         fw, fh = fft_shape if fft_shape else (0, 0)
         h = C.c_void_p()
