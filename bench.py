@@ -6,7 +6,8 @@ Workload (default): 8 exposures of 2048^2 -> 4096^2 target, Airy+wing PSFs of
 A "step" is one reconstruction iteration over all exposures.  --config 3 runs
 the low-sky case (errors at iterations 10, 20, 40) and the reference's
 FreezeMu forcing recipe; --config 4 the uneven batch of 24 exposures against
-per-exposure launches.
+per-exposure launches; --config 5 the differentiation workload (gradients
+through 10 unrolled iterations).
 
 Reported line: iterations/s of the FP32/batched form (FP32 FFTs, PSF and S_k
 products, FP64 spectrum accumulation over exposures, denominator, raw_mu test
@@ -14,7 +15,7 @@ and ratio; all exposures batched in every launch; each iteration replayed as
 a hipGraph), next to the plain FP64 GPU form and the error of the FP32/batched
 estimate against the FP64 one after the same number of iterations.
 
-Usage: python bench.py [--gpus 1] [--steps 40] [--warmup 2] [--config 2|1|3|4]
+Usage: python bench.py [--gpus 1] [--steps 40] [--warmup 2] [--config 2|1|3|4|5]
                        [--no-cpu] [--no-f64] [--no-autotune] [--profile]
 """
 from __future__ import annotations
@@ -43,7 +44,7 @@ def make_dataset(config: int):
     from updc_amd import synth
     from updc_amd.solver import Context, Exposure
     t0 = time.time()
-    ds = {1: synth.config1, 2: synth.config2, 3: synth.config3, 4: synth.config4}[config]()
+    ds = {1: synth.config1, 2: synth.config2, 3: synth.config3, 4: synth.config4, 5: synth.config5}[config]()
     log(f"synthetic {ds.name}: truth/PSFs/geometry in {time.time() - t0:.1f}s")
     t0 = time.time()
     n = len(ds.geoms)
@@ -230,6 +231,132 @@ def run_uneven(args, ds, metrics) -> dict:
     }
 
 
+def run_autograd(args, ds, metrics) -> dict:
+    """Config 5: loss = Poisson NLL of the held-out fifth exposure after K=10
+    unrolled iterations over the other four; gradients with respect to the
+    initial estimate, the PSF mixture parameters and the backgrounds through
+    the HIP operators (updc_amd.autograd), per-iteration checkpointing.
+    Reports forward+backward time (FP32 and FP64), peak memory with and
+    without checkpointing, central finite differences (h = 1e-6, FP64), the
+    FP64 torch restatement (oracle/torch_ref.py, torch.fft on the GPU) and
+    FP32 against FP64 gradients."""
+    import torch
+    from updc_amd import autograd as A
+    from updc_amd import synth
+    from updc_amd.solver import Context, Exposure
+    K = 10
+    n = len(ds.geoms)
+    true_p = synth.config5_psf_params(n)
+    start_p = true_p.copy()
+    start_p[:, :, 1] *= 1.05          # widths 5% off
+    start_p[:, 1, 0] *= 0.8           # wing amplitude 20% off
+    res = {}
+    grads = {}
+    for prec in ("mixed", "f64"):
+        c = Context(ds.W, ds.H, prec)
+        for k in range(n):
+            c.add_exposure(Exposure(ds.sci[k], ds.wht[k], ds.psfs[k], ds.background[k], ds.pedestal[k],
+                                    linkmap=ds.linkmaps[k]))
+        c.prepare()
+        ops = A.Operators(c)
+        ex = A.exposures_from_context(c, ops, ds.sci, ds.wht, ds.pedestal)
+        x0np, _ = c.initial()
+        dt, dev = ops.dtype, ops.device
+        size = ds.psfs[0].shape[0]
+
+        def loss_fn(x0, params, bs, checkpoint=True):
+            pws = [A.wrap_psf(A.gaussian_mixture(params[k], size), ops.FH, ops.FW) for k in range(n)]
+            x = A.unrolled(x0, pws[:n - 1], [bs[k] for k in range(n - 1)], K, ops, ex, checkpoint=checkpoint)
+            mu = A.forward_model(x, pws[n - 1], bs[n - 1], n - 1, ops, ex)
+            return A.poisson_nll(mu, ex.yeff[n - 1], ex.active[n - 1])
+
+        def leaves():
+            return (torch.as_tensor(x0np, dtype=dt, device=dev).requires_grad_(True),
+                    torch.as_tensor(start_p, dtype=dt, device=dev).requires_grad_(True),
+                    torch.zeros(n, dtype=dt, device=dev).requires_grad_(True))
+
+        times = []
+        for rep in range(3):
+            lv = leaves()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            L = loss_fn(*lv)
+            g = torch.autograd.grad(L, lv)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        peak = {}
+        for ck in (True, False):
+            torch.cuda.empty_cache()
+            torch.cuda.reset_peak_memory_stats(dev)
+            base = torch.cuda.memory_allocated(dev)
+            lv = leaves()
+            L2 = loss_fn(*lv, checkpoint=ck)
+            torch.autograd.grad(L2, lv)
+            torch.cuda.synchronize()
+            peak["checkpointed" if ck else "stored"] = int(torch.cuda.max_memory_allocated(dev) - base)
+        grads[prec] = [t.detach().to("cpu", torch.float64) for t in g]
+        r = dict(loss=float(L.detach()), fwd_bwd_s=round(min(times), 4), iters_per_s=round(K / min(times), 3),
+                 peak_bytes=peak, fft_shape=[ops.FW, ops.FH])
+        log(f"autograd {prec}: loss {float(L.detach()):.10g}, forward+backward {min(times) * 1e3:.1f} ms "
+            f"({K / min(times):.1f} unrolled it/s), peak memory {peak}")
+        if prec == "f64":
+            with torch.no_grad():
+                h = 1e-6
+                floor = 8 * 2.2e-16 * abs(float(L.detach())) / h
+                lv = leaves()
+                gen = torch.Generator().manual_seed(1)
+                v = torch.randn(lv[0].shape, generator=gen, dtype=torch.float64).to(dev)
+                fd_rows = []
+                f = lambda s_: float(loss_fn(lv[0] + s_ * v, lv[1], lv[2]))
+                fd = (f(h) - f(-h)) / (2 * h)
+                an = float((g[0] * v).sum())
+                fd_rows.append(("x0 direction", fd, an))
+                for idx in [(0, 0, 1), (1, 1, 0), (n - 1, 0, 2)]:
+                    e = torch.zeros_like(lv[1])
+                    e[idx] = 1.0
+                    f = lambda s_: float(loss_fn(lv[0], lv[1] + s_ * e, lv[2]))
+                    fd_rows.append((f"psf{idx}", (f(h) - f(-h)) / (2 * h), float(g[1][idx])))
+                for k in (0, n - 1):
+                    e = torch.zeros_like(lv[2])
+                    e[k] = 1.0
+                    f = lambda s_: float(loss_fn(lv[0], lv[1], lv[2] + s_ * e))
+                    fd_rows.append((f"background{k}", (f(h) - f(-h)) / (2 * h), float(g[2][k])))
+            r["finite_differences"] = [dict(param=nm, fd=fdv, analytic=anv, abs_err=abs(fdv - anv),
+                                            tol=1e-5 * abs(anv) + floor) for nm, fdv, anv in fd_rows]
+            ok = all(x["abs_err"] <= x["tol"] for x in r["finite_differences"])
+            log(f"finite differences (h=1e-6): {'pass' if ok else 'FAIL'} {r['finite_differences']}")
+            r["finite_differences_pass"] = ok
+            # the FP64 torch restatement on the GPU (torch.fft, index_add)
+            from oracle import torch_ref as R
+            m = R.RefModel(ds.W, ds.H, ops.FW, ops.FH, ds.linkmaps, device=dev)
+            lv = leaves()
+            pws = [R.wrap_psf(R.gaussian_mixture(lv[1][k], size), ops.FH, ops.FW) for k in range(n)]
+            xk = m.unrolled(lv[0], pws[:n - 1], lv[2][:n - 1], K, ex.yeff[:n - 1], ex.w[:n - 1], ex.wmask[:n - 1],
+                            ex.active[:n - 1], ex.ped[:n - 1])
+            Lr = R.poisson_nll(m.forward_model(xk, pws[n - 1], lv[2][n - 1], n - 1, ex.ped[n - 1]),
+                               ex.yeff[n - 1], ex.active[n - 1])
+            gr = [t.detach().to("cpu", torch.float64) for t in torch.autograd.grad(Lr, lv)]
+            rel = [float((a - b).norm() / b.norm()) for a, b in zip(grads["f64"], gr)]
+            r["torch_reference"] = dict(loss_rel=abs(float(L.detach()) - float(Lr.detach())) / abs(float(Lr.detach())), grad_rel=rel)
+            log(f"FP64 torch restatement: {r['torch_reference']}")
+        res[prec] = r
+        del ops, ex
+        c.close()
+    rel32 = [float((a - b).norm() / b.norm()) for a, b in zip(grads["mixed"], grads["f64"])]
+    log(f"FP32 vs FP64 gradients (x0, psf params, backgrounds): {rel32}")
+    its = res["mixed"]["iters_per_s"]
+    return {
+        "metric": METRIC, "value": its, "unit": "iters/s", "n_gpus": 1, "steps": K, "warmup": 1,
+        "ms_per_step": round(1e3 / its, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 accumulation/denominator/ratio)", "data": "synthetic",
+        "config": {"workload": f"config5 (differentiation): {n - 1}x{ds.geoms[0].iw}^2 exposures + 1 held out -> "
+                               f"{ds.W}^2 target, PSF {ds.psfs[0].shape[0]}^2 2-component mixtures, "
+                               f"K={K} unrolled iterations forward+backward, per-iteration checkpointing",
+                   "fp32": res["mixed"], "fp64": res["f64"], "fp32_vs_fp64_grad_rel": rel32},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def stage_profile(c, iters: int) -> dict:
     """Per-kernel average device times over a timed region of `iters` eager
     (non-graph) iterations: HIP events recorded around every launch on the
@@ -292,8 +419,8 @@ def main() -> int:
             f"-> {2 * nbytes / (ms.value * 1e-3) / 1e9:.1f} GB/s")
     ds = make_dataset(args.config)
     K, Wm = args.steps, args.warmup
-    if args.config in (3, 4):
-        line = (run_stress if args.config == 3 else run_uneven)(args, ds, metrics)
+    if args.config in (3, 4, 5):
+        line = {3: run_stress, 4: run_uneven, 5: run_autograd}[args.config](args, ds, metrics)
         print(json.dumps(line), flush=True)
         return 0
 

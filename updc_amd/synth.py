@@ -227,6 +227,41 @@ def config4(target: int = 6144, nsrc: int = 11250) -> Dataset:
     return Dataset(target, target, geoms, psfs, truth, [0.0] * len(sizes), 10.0, 0.01, None, 40, "config4")
 
 
+def config5_psf_params(n: int = 5) -> np.ndarray:
+    """(n, 2, 4) true PSF mixture parameters (a, s, dx, dy) of config 5:
+    a core of sigma ~0.9 and a wider component of sigma ~2, amplitude 0.2."""
+    g = rng_for(5, "psf")
+    p = np.zeros((n, 2, 4))
+    for k in range(n):
+        p[k, 0] = [1.0, float(g.uniform(0.8, 1.0)), float(g.normal(0, 0.1)), float(g.normal(0, 0.1))]
+        p[k, 1] = [0.2, float(g.uniform(1.8, 2.4)), float(g.normal(0, 0.2)), float(g.normal(0, 0.2))]
+    return p
+
+
+def mixture_image(params: np.ndarray, size: int) -> np.ndarray:
+    """Normalised float32 image of a (ncomp, 4) mixture, the same formula as
+    updc_amd.autograd.gaussian_mixture."""
+    c = (size - 1) / 2.0
+    ax = np.arange(size, dtype=np.float64) - c
+    y, x = ax[:, None], ax[None, :]
+    img = sum(a * np.exp(-((x - dx) ** 2 + (y - dy) ** 2) / (2.0 * s * s)) for a, s, dx, dy in params)
+    return (img / img.sum()).astype(np.float32)
+
+
+def config5(target: int = 2048, det: int = 1024, psf_size: int = 255, nsrc: int = 1250) -> Dataset:
+    """Differentiation workload: N=4 exposures of 1024^2 plus a held-out fifth
+    at 2x scale on a 2048^2 target, 255^2 PSFs that are 2-component Gaussian
+    mixtures (config5_psf_params), 10 unrolled iterations."""
+    g = rng_for(5, "geometry")
+    c = (target - 1) / 2.0
+    geoms = [Geometry(det, det, target / det, float(g.uniform(-0.5, 0.5)),
+                      (float(g.uniform(0, 10)), float(g.uniform(0, 10))), g.normal(0.0, 1e-3, (2, 10)), (c, c))
+             for _ in range(5)]
+    psfs = [mixture_image(p, psf_size) for p in config5_psf_params(5)]
+    truth = truth_image(target, target, nsrc, 0.3, rng_for(5, "truth"), sersic_fraction=0.5)
+    return Dataset(target, target, geoms, psfs, truth, [0.0] * 5, 10.0, 0.01, None, 10, "config5")
+
+
 def small_config(n_exp=2, det=48, target=96, psf=9, seed=7, rotation=3.0, dither=3.0, distort=0.0,
                  nsrc=20, sky=0.5, mask_fraction=0.02, read_noise=2.0) -> Dataset:
     """Tiny dataset for oracle-sized parity tests."""
