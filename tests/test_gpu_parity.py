@@ -288,3 +288,38 @@ def test_freeze_forcing_invariants():
         for a, b in zip(mine, mine[1:]):
             assert b[3] == a[3] + b[2]
     c.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "mixed", "f64"])
+def test_split_adjoint_matches_fused(small_ds, prec):
+    """The adjoint gather as its own launch feeds the same values to the same
+    row transform: identical estimates and FreezeMu rows."""
+    print(f"case: split adjoint vs fused {prec}", flush=True)
+    out = []
+    for split in (False, True):
+        c = _ctx(small_ds, prec, graph=True, use_vertices=False)
+        c.set_split_adjoint(split)
+        c.prepare()
+        rows, _ = c.iterate(3)
+        out.append((c.estimate(f64=True), [tuple(r[:4]) for r in rows]))
+        c.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+def test_initial_estimate_survives_iterations_and_runs_repeat(small_ds):
+    """updc_get_initial_f64 returns the set-up estimate after iterating, and a
+    context reset to it (estimate and FreezeMu masks) repeats a run exactly."""
+    print("case: initial estimate kept; repeated runs on one context", flush=True)
+    c = _ctx(small_ds, "mixed", graph=True, use_vertices=False)
+    c.prepare()
+    x0, _ = c.initial()
+    out = []
+    for _ in range(2):
+        c.set_estimate(x0)
+        c.reset_frozen()
+        rows, _ = c.iterate(3)
+        out.append((c.estimate(f64=True), [tuple(r[:4]) for r in rows]))
+        assert np.array_equal(c.initial()[0], x0)
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+    c.close()

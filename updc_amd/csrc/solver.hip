@@ -226,6 +226,7 @@ struct PipeBase {
     virtual void fft_c2r(Ctx &c, const double *spec, double *img) = 0;
     virtual void get_frozen(Ctx &c, int k, uint8_t *out) = 0;
     virtual void reset_frozen(Ctx &c) = 0;
+    virtual void invalidate_graph() = 0;  // after a flag change that alters the launches
     // device-pointer operators (element type TF), on the caller's stream
     virtual void op_r2c(Ctx &c, const void *img, int rows, int cols, int pitch, void *spec, hipStream_t st) = 0;
     virtual void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double scale,
@@ -273,6 +274,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     DevBuf<ExpDev> d_ex;
     std::vector<ExpDev> h_ex;
     DevBuf<TF> sci, w, wr, blur, x, C, cov;
+    DevBuf<TF> x_init;  // the initial estimate, kept for updc_get_initial_f64
     DevBuf<TA> D;
     DevBuf<uint8_t> frozen;
     DevBuf<Cx<TF>> psf_spec, spec_x, spec_k, spec_tmp;
@@ -454,6 +456,8 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(hipMemcpyAsync(&ebits, err.p, 4, hipMemcpyDeviceToHost, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         if (ebits & UPDC_E_DRIZZLE) throw Error(UPDC_ERR_NUMERIC, "non-finite signed initial estimate");
+        x_init.alloc(nout);
+        HIPCHK(hipMemcpyAsync(x_init.p, x.p, nout * sizeof(TF), hipMemcpyDeviceToDevice, c.st));
 
         // denominator
         D.alloc(nout);
@@ -572,16 +576,30 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK((gather_freeze<TF, TA>(d_ex.p, 0, n, max_nin, blur.p, nout, sci.p, w.p, frozen.p, wr.p, diag_cur.p,
                                       idiag_cur.p, c.st)));
         mark(4);
-        HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
-        mark(5);
+        int m = 5;
+        if (c.flags & UPDC_FLAG_SPLIT_ADJOINT) {
+            // S_k^T wr_k for every exposure into the (now dead) blur images, then
+            // the plain row R2C of those images
+            HIPCHK(csc_image<TF>(d_ex.p, n, wr.p, nout, blur.p, c.st));
+            mark(m++);
+            HIPCHK(fft.row_r2c(ROW_PLAIN, blur.p, nout, c.W, c.W, nullptr, nullptr, 0, spec_k.p, kstride, c.H, c.H, n,
+                               c.st));
+        } else {
+            HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
+        }
+        mark(m++);
         HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p,
                                         fft.spec_elems(s.FH), c.H, c.H, n, c.st));
-        mark(6);
+        mark(m++);
         HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
-        mark(7);
+        mark(m++);
         HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
-        mark(8);
+        mark(m++);
         HIPCHK(commit_diag(diag_cur.p, n, idiag_cur.p, diag_hist.p, idiag_hist.p, hist_cap, ctr.p, c.st));
+    }
+    void invalidate_graph() override {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
     }
 
     void iterate(Ctx &c, int niter, updc_freeze_row *rows, updc_iter_info *info) override {
@@ -597,7 +615,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(fill<unsigned int>(ctr.p, 0u, 1, c.st));
         const bool use_graph = (c.flags & UPDC_FLAG_GRAPH) != 0;
         std::vector<hipEvent_t> ev;
-        const int nev = 9;
+        const int nev = 10;
         hipEvent_t it0, it1;
         HIPCHK(hipEventCreate(&it0));
         HIPCHK(hipEventCreate(&it1));
@@ -637,11 +655,16 @@ template <class TF, class TA> struct Pipe : PipeBase {
         float total_ms = 0;
         HIPCHK(hipEventElapsedTime(&total_ms, it0, it1));
         const bool plain = (c.flags & UPDC_FLAG_PER_EXPOSURE) != 0;
-        const int nst = plain ? 4 : 8;
-        stage_names = plain ? std::vector<std::string>{"plain_xhat", "plain_exposures", "plain_adjoint_inverse",
-                                                       "update"}
-                            : std::vector<std::string>{"row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze",
-                                                       "row_r2c_gather", "col_adj_accum", "row_c2r_corr", "update"};
+        const bool split = (c.flags & UPDC_FLAG_SPLIT_ADJOINT) != 0;
+        if (plain)
+            stage_names = {"plain_xhat", "plain_exposures", "plain_adjoint_inverse", "update"};
+        else if (split)
+            stage_names = {"row_r2c_x",     "col_fwd_bcast", "row_c2r_blur",  "gather_freeze", "adjoint_image",
+                           "row_r2c_adj",   "col_adj_accum", "row_c2r_corr",  "update"};
+        else
+            stage_names = {"row_r2c_x",      "col_fwd_bcast", "row_c2r_blur", "gather_freeze",
+                           "row_r2c_gather", "col_adj_accum", "row_c2r_corr", "update"};
+        const int nst = (int)stage_names.size();
         stage_ms.assign(nst + 1, 0.0);
         stage_n.assign(nst + 1, 0);
         if (!use_graph) {
@@ -745,7 +768,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     }
     void get_denominator(Ctx &c, double *out) override { to_host_f64(D.p, c.nout(), out, c); }
     void get_initial(Ctx &c, double *est, double *covo) override {
-        if (est) to_host_f64(x.p, c.nout(), est, c);
+        if (est) to_host_f64(x_init.p, c.nout(), est, c);
         if (covo) to_host_f64(cov.p, c.nout(), covo, c);
     }
     // column-major (pitch rp(rows)) device spectrum -> row-major rows x NC host array
@@ -1299,6 +1322,8 @@ int updc_ctx_destroy(updc_ctx *ctx) {
 int updc_ctx_set_flags(updc_ctx *ctx, unsigned flags) {
     return guarded([&] {
         need(ctx != nullptr, UPDC_ERR_ARG, "null context");
+        const unsigned shape = UPDC_FLAG_PER_EXPOSURE | UPDC_FLAG_SPLIT_ADJOINT;
+        if (((ctx->flags ^ flags) & shape) && ctx->pipe) ctx->pipe->invalidate_graph();
         ctx->flags = flags;
     });
 }

@@ -909,6 +909,46 @@ hipError_t copy_ref(const void *src, void *dst, size_t bytes, hipStream_t st) {
 }
 
 // ---- explicit instantiations
+// Batched adjoint image: out[k * nout + t] = sum over the CSC entries of
+// target t of fma(frac, v[pix_k + p], .) in ascending p (the reference's
+// scatter order, updc_core.c:1171-1176), for every exposure k of the batch
+// (blockIdx.y).  Grid-stride over targets; the first CSCI_U entries of a
+// target and their v loads are issued together.
+static constexpr int CSCI_NT = 256, CSCI_U = 4;
+template <class TF>
+__global__ __launch_bounds__(CSCI_NT) void k_csc_image(const ExpDev *__restrict__ ex, const TF *__restrict__ v,
+                                                       size_t nout, TF *__restrict__ out) {
+    const ExpDev &e = ex[blockIdx.y];
+    const TF *vk = v + e.pix;
+    TF *ok = out + (size_t)blockIdx.y * nout;
+    for (size_t t = (size_t)blockIdx.x * CSCI_NT + threadIdx.x; t < nout; t += (size_t)gridDim.x * CSCI_NT) {
+        const uint32_t a = e.toff[t], n = e.toff[t + 1] - a;
+        uint2 en[CSCI_U];
+        TF wv[CSCI_U];
+#pragma unroll
+        for (int j = 0; j < CSCI_U; ++j) en[j] = (uint32_t)j < n ? e.tent[a + j] : make_uint2(0u, 0u);
+#pragma unroll
+        for (int j = 0; j < CSCI_U; ++j) wv[j] = (uint32_t)j < n ? vk[en[j].x] : (TF)0;
+        TF acc = (TF)0;
+#pragma unroll
+        for (int j = 0; j < CSCI_U; ++j)
+            if ((uint32_t)j < n) acc = fmat((TF)__uint_as_float(en[j].y), wv[j], acc);
+        for (uint32_t j = CSCI_U; j < n; ++j) {
+            const uint2 x = e.tent[a + j];
+            acc = fmat((TF)__uint_as_float(x.y), vk[x.x], acc);
+        }
+        ok[t] = acc;
+    }
+}
+template <class TF> hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, hipStream_t st) {
+    if (nexp <= 0) return hipSuccess;
+    const unsigned gx = (unsigned)std::min<size_t>((nout + CSCI_NT - 1) / CSCI_NT, (size_t)std::max(1, 16384 / nexp));
+    hipLaunchKernelGGL((k_csc_image<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
+    return hipGetLastError();
+}
+template hipError_t csc_image<float>(const ExpDev *, int, const float *, size_t, float *, hipStream_t);
+template hipError_t csc_image<double>(const ExpDev *, int, const double *, size_t, double *, hipStream_t);
+
 template hipError_t csc_apply<float>(const ExpDev *, const float *, size_t, float *, hipStream_t);
 template hipError_t csc_apply<double>(const ExpDev *, const double *, size_t, double *, hipStream_t);
 template hipError_t csc_count<float>(const ExpDev *, uint32_t, const float *, uint32_t *, hipStream_t);

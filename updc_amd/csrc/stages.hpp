@@ -83,5 +83,8 @@ hipError_t commit_diag(ExpDiag *cur, int nexp, IterDiag *icur, ExpDiag *hist, It
                        unsigned int *ctr, hipStream_t st);
 // out[t] = (S_k^T v)[t] for one exposure through the CSC gather (H*W targets).
 template <class TF> hipError_t csc_apply(const ExpDev *e_dev, const TF *v, size_t nout, TF *out, hipStream_t st);
+// All exposures of the batch: out[k * nout + t] = (S_k^T v_k)[t] over the CSC.
+template <class TF>
+hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, hipStream_t st);
 
 }  // namespace updc

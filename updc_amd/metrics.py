@@ -47,6 +47,9 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
         "gather_freeze": lm + N * nout * e + pix * (3 * e + 2),
         # CSC + wr + spectrum rows
         "row_r2c_gather": csc + pix * e + spec_k,
+        # split form: CSC + wr in, N adjoint images out; then those images in, spectra out
+        "adjoint_image": csc + pix * e + N * nout * e,
+        "row_r2c_adj": N * nout * e + spec_k,
         "col_adj_accum": spec_k + psf + spec_x,
         "row_c2r_corr": spec_x + nout * e,
         "update": nout * (3 * e + e_d),

@@ -30,8 +30,8 @@ from collections import defaultdict
 from pathlib import Path
 
 # iteration stage names, in launch order (solver.hip launch_iteration)
-STAGES = ("row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze", "row_r2c_gather", "col_adj_accum",
-          "row_c2r_corr", "update")
+STAGES = ("row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze", "row_r2c_gather", "adjoint_image",
+          "row_r2c_adj", "col_adj_accum", "row_c2r_corr", "update")
 
 
 def _template_args(name: str) -> list[str]:
@@ -46,7 +46,7 @@ def classify(name: str, grid_y: int) -> str | None:
         mode = _template_args(name)[-1]
         if mode == "1":
             return "row_r2c_gather"
-        return "row_r2c_x" if grid_y == 1 else None
+        return "row_r2c_x" if grid_y == 1 else "row_r2c_adj"
     if "k_row_c2r<" in name:
         return "row_c2r_corr" if grid_y == 1 else "row_c2r_blur"
     if "k_col_e<" in name or "k_col<" in name:
@@ -55,6 +55,8 @@ def classify(name: str, grid_y: int) -> str | None:
         return {"2": "col_fwd_bcast", "3": "col_adj_accum"}.get(mode)
     if "k_gather<" in name:
         return "gather_freeze"
+    if "k_csc_image<" in name:
+        return "adjoint_image"
     if "k_update<" in name:
         return "update"
     return None

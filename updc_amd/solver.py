@@ -14,7 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import FLAG_AUTOTUNE, FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FreezeRow, IterInfo, check
+from ._lib import (FLAG_AUTOTUNE, FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FLAG_SPLIT_ADJOINT, FreezeRow,
+                   IterInfo, check)
 
 
 def _f32(a) -> np.ndarray:
@@ -108,6 +109,12 @@ class Context:
         n, cached = C.c_int(), C.c_int()
         check(self.lib.updc_get_autotune(self.h, w, h, e, ms, cap, C.byref(n), C.byref(cached)))
         return [(w[i], h[i], e[i], ms[i]) for i in range(min(n.value, cap))], bool(cached.value)
+
+    def set_split_adjoint(self, on: bool) -> None:
+        """Adjoint gather as its own launch followed by a plain row R2C (True),
+        or fused into the row R2C (False)."""
+        self.flags = (self.flags | FLAG_SPLIT_ADJOINT) if on else (self.flags & ~FLAG_SPLIT_ADJOINT)
+        check(self.lib.updc_ctx_set_flags(self.h, self.flags))
 
     def set_graph(self, on: bool) -> None:
         self.flags = (self.flags | FLAG_GRAPH) if on else (self.flags & ~FLAG_GRAPH)
