@@ -150,7 +150,11 @@ __device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int cfix, i
     static_assert(NB >= 1, "VMAX too small for radix");
     const bool one = cfix >= 0;
     const int lm = (1 << lg) - 1;
-    const int LR = L / R;
+    // Redefine L/R opaquely (it stays in an SGPR): otherwise the compiler
+    // hoists every radix case's per-thread LDS addresses out of the stage
+    // loop and spills them to scratch.
+    int LR = L / R;
+    asm volatile("" : "+s"(LR));
     const int nb = one ? LR : (LR << lg);
     Cx<T> v[NB][R];
     const int tid = threadIdx.x;

@@ -21,11 +21,16 @@ namespace updc {
 // entries of each loaded together (further entries, rare, one at a time).
 static constexpr int RG_GB = 8, RG_GU = 3;
 
+// Minimum waves per SIMD of the row kernels: FP32 keeps two 512-thread
+// workgroups per CU (<= 128 VGPRs), so one workgroup's loads overlap the
+// other's transform; FP64 keeps one.
+template <class T, int NT> constexpr int row_min_waves() { return sizeof(T) == 4 ? 2048 / NT : 1024 / NT; }
+
 // Row R2C: RB consecutive rows per workgroup (interleaved in LDS, row j at
 // slot (n << LG) + j), one half-length complex FFT per row, then the real
 // split written column-major, so RB rows of one column land contiguously.
 template <class T, int NT, int VMAX, int RB, int MODE>
-__global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restrict__ post, const T *__restrict__ in,
+__global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen f, const Cx<T> *__restrict__ post, const T *__restrict__ in,
                                                 size_t in_bstride, int in_pitch, int win,
                                                 const ExpDev *__restrict__ ex, const T *__restrict__ wr, int W,
                                                 Cx<T> *__restrict__ out, size_t out_bstride, int RP, int rows) {
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(NT) void k_row_r2c(FftLen f, const Cx<T> *__restric
 // so the merge needs no second buffer), transformed together, and written
 // cropped and scaled, row-major.
 template <class T, int NT, int VMAX, int RB>
-__global__ __launch_bounds__(NT) void k_row_c2r(FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in,
+__global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r(FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in,
                                                 size_t in_bstride, int RP, T *__restrict__ out, size_t out_bstride,
                                                 int out_pitch, int wout, T scale, int rows) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
