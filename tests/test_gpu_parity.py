@@ -323,3 +323,24 @@ def test_initial_estimate_survives_iterations_and_runs_repeat(small_ds):
         assert np.array_equal(c.initial()[0], x0)
     assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
     c.close()
+
+
+def test_odd_target_size_matches_oracle():
+    """An odd target width (rows of odd pitch: the row passes take the scalar
+    pair path) and odd height, FP64 against the FP64 oracle."""
+    from oracle import oracle as O
+    from updc_amd import synth
+    print("case: odd target 95 x 97, f64 vs oracle", flush=True)
+    ds = synth.small_config(n_exp=2, det=44, target=95, psf=7, seed=21)
+    ds.H = 97
+    ds.truth = np.pad(ds.truth, ((0, 2), (0, 0)), mode="edge")
+    ds = oracle_dataset(ds)
+    ref = O.RefSolver(ds.W, ds.H, oracle_exposures(ds), "f64")
+    c = _ctx(ds, "f64", use_vertices=False)
+    c.prepare()
+    rows_ref, _ = ref.iterate(3)
+    rows, _ = c.iterate(3)
+    st = O.compare_stats(ref.x, c.estimate(f64=True))
+    assert st["norm_linf"] < 1e-11, st
+    assert [tuple(r[:4]) for r in rows] == [tuple(r[:4]) for r in rows_ref]
+    c.close()

@@ -21,6 +21,36 @@ namespace updc {
 // entries of each loaded together (further entries, rare, one at a time).
 static constexpr int RG_GB = 8, RG_GU = 3;
 
+// A real row's (x[2n], x[2n+1]) pair as one 8-byte (FP32) or 16-byte (FP64)
+// access when `vec` (the host checked an even pitch and batch stride and an
+// aligned base, so every pair is aligned), else two scalar accesses.
+template <class T> struct Pair;
+template <> struct Pair<float> { typedef float2 type; };
+template <> struct Pair<double> { typedef double2 type; };
+template <class T> __device__ __forceinline__ Cx<T> load_pair(const T *row, int c, int lim, bool vec) {
+    if (!vec) return cx<T>(c < lim ? row[c] : (T)0, c + 1 < lim ? row[c + 1] : (T)0);
+    if (c + 1 < lim) {
+        const typename Pair<T>::type v = *reinterpret_cast<const typename Pair<T>::type *>(row + c);
+        return cx<T>(v.x, v.y);
+    }
+    return cx<T>(c < lim ? row[c] : (T)0, (T)0);
+}
+template <class T> __device__ __forceinline__ void store_pair(T *row, int c, int lim, T a, T b, bool vec) {
+    if (!vec) {
+        if (c < lim) row[c] = a;
+        if (c + 1 < lim) row[c + 1] = b;
+        return;
+    }
+    if (c + 1 < lim) {
+        typename Pair<T>::type v;
+        v.x = a;
+        v.y = b;
+        *reinterpret_cast<typename Pair<T>::type *>(row + c) = v;
+    } else if (c < lim) {
+        row[c] = a;
+    }
+}
+
 // Minimum waves per SIMD of the row kernels: FP32 keeps two 512-thread
 // workgroups per CU (<= 128 VGPRs), so one workgroup's loads overlap the
 // other's transform; FP64 keeps one.
@@ -33,7 +63,7 @@ template <class T, int NT, int VMAX, int RB, int MODE>
 __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen f, const Cx<T> *__restrict__ post, const T *__restrict__ in,
                                                 size_t in_bstride, int in_pitch, int win,
                                                 const ExpDev *__restrict__ ex, const T *__restrict__ wr, int W,
-                                                Cx<T> *__restrict__ out, size_t out_bstride, int RP, int rows) {
+                                                Cx<T> *__restrict__ out, size_t out_bstride, int RP, int rows, int vec) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
     static_assert((1 << LG) == RB, "RB must be 1, 2, 4 or 8");
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -50,9 +80,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen
             const T *row = in + (size_t)b * in_bstride + (size_t)r * in_pitch;
             for (int n = threadIdx.x; n < M; n += NT) {
                 const int c = 2 * n;
-                const T x0 = (r < rows && c < win) ? row[c] : (T)0;
-                const T x1 = (r < rows && c + 1 < win) ? row[c + 1] : (T)0;
-                z[(n << LG) + j] = cx<T>(x0, x1);
+                z[(n << LG) + j] = r < rows ? load_pair(row, c, win, vec != 0) : cx<T>((T)0, (T)0);
             }
         }
     } else {
@@ -133,7 +161,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen
 template <class T, int NT, int VMAX, int RB>
 __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r(FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in,
                                                 size_t in_bstride, int RP, T *__restrict__ out, size_t out_bstride,
-                                                int out_pitch, int wout, T scale, int rows) {
+                                                int out_pitch, int wout, T scale, int rows, int vec) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);  // (M + 1) * RB slots
@@ -170,8 +198,145 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r(FftLen
         for (int n = threadIdx.x; n < M; n += NT) {
             const int c = 2 * n;
             const Cx<T> v = z[(n << LG) + j];
-            if (c < wout) orow[c] = v.re * scale;
-            if (c + 1 < wout) orow[c + 1] = v.im * scale;
+            store_pair(orow, c, wout, v.re * scale, v.im * scale, vec != 0);
+        }
+    }
+}
+
+// ---- persistent, software-pipelined row passes ------------------------------
+// One workgroup per CU (the LDS holds one tile of RB rows) walks a range of
+// tiles; tile t = (exposure t / nblk, row block t % nblk).  XCD x owns the
+// contiguous tiles [x * tper, (x + 1) * tper), taken round robin by its
+// gridDim.x / 8 workgroups, so the row blocks sharing a spectrum column's
+// 128-byte lines run on one XCD at the same time.  The next tile's input is
+// loaded into registers while the current tile is transformed, which hides
+// the load latency that a lone workgroup per CU cannot otherwise cover.
+__device__ __forceinline__ void row_tiles(int ntiles, int &t, int &t_end, int &step) {
+    const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+    step = gridDim.x >> 3;
+    const int tper = (ntiles + 7) >> 3;
+    t = xcd * tper + l;
+    t_end = min(ntiles, (xcd + 1) * tper);
+}
+
+// C2R (as k_row_c2r).  PFM complex values per thread cover (M + 1) * RB slots.
+template <class T, int NT, int VMAX, int RB, int PFM>
+__global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
+    FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in, size_t in_bstride, int RP,
+    T *__restrict__ out, size_t out_bstride, int out_pitch, int wout, T scale, int rows, int batch, int vec) {
+    constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
+    const int M = f.L, nblk = (rows + RB - 1) / RB, nsl = (M + 1) << LG;
+    int t, t_end, step;
+    row_tiles(nblk * batch, t, t_end, step);
+    Cx<T> pre[PFM];
+    // slot idx = tid + m NT holds column v = idx / RB, row j = idx % RB; NT is a
+    // multiple of RB, so j is fixed per thread and v advances by NT / RB per slot
+    const int jt = threadIdx.x & (RB - 1);
+    const size_t vstep = (size_t)(NT >> LG) * RP;
+    auto load = [&](int tt) {
+        const int b = tt / nblk, r0 = (tt - b * nblk) * RB;
+        const Cx<T> *p0 = in + (size_t)b * in_bstride + (size_t)(threadIdx.x >> LG) * RP + r0 + jt;
+        const bool rok = r0 + jt < rows;
+#pragma unroll
+        for (int m = 0; m < PFM; ++m) {
+            const int idx = threadIdx.x + m * NT;
+            pre[m] = (idx < nsl && rok) ? p0[m * vstep] : cx<T>((T)0, (T)0);
+        }
+    };
+    if (t < t_end) load(t);
+    for (; t < t_end; t += step) {
+        const int b = t / nblk, r0 = (t - b * nblk) * RB;
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int m = 0; m < PFM; ++m) {
+            const int idx = threadIdx.x + m * NT;
+            if (idx < nsl) z[idx] = pre[m];
+        }
+        if (t + step < t_end) load(t + step);
+        __syncthreads();
+        // Merge: C[v] = E[v] + i O[v], E = Y[v] + Y*[M-v], O = (Y[v] - Y*[M-v]) exp(+2 pi i v/FW).
+        for (int idx = threadIdx.x; idx < ((M / 2 + 1) << LG); idx += NT) {
+            const int v = idx >> LG, j = idx & (RB - 1);
+            const int u = M - v;
+            const Cx<T> yv = z[(v << LG) + j], yu = z[(u << LG) + j];
+            const Cx<T> e1 = yv + cconj(yu), o1 = cmul(yv - cconj(yu), cconj(post[v]));
+            const Cx<T> cv = cx<T>(e1.re - o1.im, e1.im + o1.re);
+            if (v != 0 && u != v) {
+                const Cx<T> e2 = yu + cconj(yv), o2 = cmul(yu - cconj(yv), cconj(post[u]));
+                z[(u << LG) + j] = cx<T>(e2.re - o2.im, e2.im + o2.re);
+            }
+            z[(v << LG) + j] = cv;
+        }
+        __syncthreads();
+        fft_lds<true, T, NT, VMAX>(z, f, LG);
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            if (r0 + j >= rows) continue;
+            T *orow = out + (size_t)b * out_bstride + (size_t)(r0 + j) * out_pitch;
+            for (int n = threadIdx.x; n < M; n += NT) {
+                const int c = 2 * n;
+                const Cx<T> v = z[(n << LG) + j];
+                store_pair(orow, c, wout, v.re * scale, v.im * scale, vec != 0);
+            }
+        }
+    }
+}
+
+// R2C of real rows (as k_row_r2c<..., ROW_PLAIN>).  PFR (x[2n], x[2n+1]) pairs
+// per thread and row cover M pairs.
+template <class T, int NT, int VMAX, int RB, int PFR>
+__global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c_p(
+    FftLen f, const Cx<T> *__restrict__ post, const T *__restrict__ in, size_t in_bstride, int in_pitch, int win,
+    Cx<T> *__restrict__ out, size_t out_bstride, int RP, int rows, int batch, int vec) {
+    constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
+    const int M = f.L, nblk = (rows + RB - 1) / RB;
+    int t, t_end, step;
+    row_tiles(nblk * batch, t, t_end, step);
+    Cx<T> pre[RB][PFR];
+    auto load = [&](int tt) {
+        const int b = tt / nblk, r0 = (tt - b * nblk) * RB;
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            const int r = r0 + j;
+            const T *row = in + (size_t)b * in_bstride + (size_t)r * in_pitch;
+#pragma unroll
+            for (int m = 0; m < PFR; ++m) {
+                const int n = threadIdx.x + m * NT, c = 2 * n;
+                const bool ok = n < M && r < rows;
+                pre[j][m] = ok ? load_pair(row, c, win, vec != 0) : cx<T>((T)0, (T)0);
+            }
+        }
+    };
+    if (t < t_end) load(t);
+    const T half = (T)0.5;
+    for (; t < t_end; t += step) {
+        const int b = t / nblk, r0 = (t - b * nblk) * RB;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RB; ++j)
+#pragma unroll
+            for (int m = 0; m < PFR; ++m) {
+                const int n = threadIdx.x + m * NT;
+                if (n < M) z[(n << LG) + j] = pre[j][m];
+            }
+        if (t + step < t_end) load(t + step);
+        __syncthreads();
+        fft_lds<false, T, NT, VMAX>(z, f, LG);
+        // Split: X[v] = (Z[v] + Z*[M-v])/2 - i/2 exp(-2 pi i v/FW) (Z[v] - Z*[M-v]), v in [0, M].
+        Cx<T> *o = out + (size_t)b * out_bstride;
+        for (int idx = threadIdx.x; idx < ((M + 1) << LG); idx += NT) {
+            const int v = idx >> LG, j = idx & (RB - 1);
+            if (r0 + j >= rows) continue;
+            const Cx<T> zv = z[((v == M ? 0 : v) << LG) + j];
+            const Cx<T> zm = cconj(z[((v == 0 ? 0 : M - v) << LG) + j]);
+            const Cx<T> fe = cscale(zv + zm, half);
+            const Cx<T> d = zv - zm;
+            const Cx<T> fo = cx<T>(d.im * half, -d.re * half);
+            o[(size_t)v * RP + r0 + j] = fe + cmul(post[v], fo);
         }
     }
 }
@@ -411,8 +576,10 @@ __global__ void k_spec(const Cx<T> *__restrict__ a, const Cx<T> *__restrict__ ps
 // waves; the budgets keep the column kernels free of spills.  Row
 // workgroups hold RB rows (RB x M points) to write RB-element column runs.
 template <class T> struct RowCfg;
-template <> struct RowCfg<float> { static constexpr int NT = 1024, VMAX = 16, RB = 4; };
-template <> struct RowCfg<double> { static constexpr int NT = 512, VMAX = 16, RB = 2; };
+// PFM / PFR: prefetch registers of the persistent row kernels; shapes beyond
+// them (FP32: M > 3071) use the one-tile-per-workgroup kernels.
+template <> struct RowCfg<float> { static constexpr int NT = 1024, VMAX = 16, RB = 4, PFM = 12, PFR = 4; };
+template <> struct RowCfg<double> { static constexpr int NT = 512, VMAX = 16, RB = 2, PFM = 16, PFR = 8; };
 template <class T> struct ColCfg;
 // PF: PSF prefetch in the edge column kernel (FP64 would spill with it).
 template <> struct ColCfg<float> { static constexpr int NT = 512, VMAX = 16, XMAX = 16; static constexpr bool PF = true; };
@@ -523,6 +690,17 @@ template <class T> static Cx<T> *upload_roots(int L, int count, int sign, std::s
 template <class T, int MODE> static void *row_r2c_fn() {
     return (void *)k_row_r2c<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, MODE>;
 }
+// Pair accesses (load_pair / store_pair) need every (2n, 2n+1) pair of every
+// row and batch image aligned to 2 elements.
+template <class T> static int pairs_aligned(const void *p, int pitch, size_t bstride) {
+    return p && ((uintptr_t)p % (2 * sizeof(T))) == 0 && pitch % 2 == 0 && bstride % 2 == 0;
+}
+template <class T> static void *row_c2r_p_fn() {
+    return (void *)k_row_c2r_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM>;
+}
+template <class T> static void *row_r2c_p_fn() {
+    return (void *)k_row_r2c_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFR>;
+}
 template <class T> static void *row_c2r_fn() {
     return (void *)k_row_c2r<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB>;
 }
@@ -574,7 +752,18 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     col.tw = tw_col;
     const int rlds = (int)row_lds<T>(s.M);
     const int col_lds = (int)((size_t)FH * sizeof(Cx<T>));
-    void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>()};
+    void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_c2r_p_fn<T>(),
+                       row_r2c_p_fn<T>()};
+    // persistent row passes: one workgroup per CU, prefetch registers must cover a tile
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+            return "cannot query the device";
+        row_grid = std::max(8, prop.multiProcessorCount / 8 * 8);
+        constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT;
+        row_persistent = row_persistent && (s.M + 1) * RB <= RowCfg<T>::PFM * NT && s.M <= RowCfg<T>::PFR * NT;
+    }
     for (void *fn : fns_row)
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, rlds) != hipSuccess)
             return "cannot raise LDS limit for the row pass";
@@ -609,12 +798,18 @@ hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pit
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     const size_t lds = row_lds<T>(s.M);
     dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
+    const int vec = pairs_aligned<T>(in, in_pitch, in_bstride);
+    if (mode == ROW_PLAIN && row_persistent) {
+        hipLaunchKernelGGL((k_row_r2c_p<T, NT, VMAX, RB, RowCfg<T>::PFR>), dim3(row_grid), dim3(NT), lds, st, row, post,
+                           in, in_bstride, in_pitch, win, out, out_bstride, rp(rp_rows), rows, batch, vec);
+        return hipGetLastError();
+    }
     if (mode == ROW_PLAIN)
         hipLaunchKernelGGL((k_row_r2c<T, NT, VMAX, RB, ROW_PLAIN>), grid, dim3(NT), lds, st, row, post, in, in_bstride,
-                           in_pitch, win, ex, wr, W, out, out_bstride, rp(rp_rows), rows);
+                           in_pitch, win, ex, wr, W, out, out_bstride, rp(rp_rows), rows, vec);
     else
         hipLaunchKernelGGL((k_row_r2c<T, NT, VMAX, RB, ROW_GATHER>), grid, dim3(NT), lds, st, row, post, in, in_bstride,
-                           in_pitch, win, ex, wr, W, out, out_bstride, rp(rp_rows), rows);
+                           in_pitch, win, ex, wr, W, out, out_bstride, rp(rp_rows), rows, 0);
     return hipGetLastError();
 }
 
@@ -624,9 +819,16 @@ hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *
     if (rows <= 0 || batch <= 0) return hipSuccess;
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     const size_t lds = row_lds<T>(s.M);
+    const int vec = pairs_aligned<T>(out, out_pitch, out_bstride);
+    if (row_persistent) {
+        hipLaunchKernelGGL((k_row_c2r_p<T, NT, VMAX, RB, RowCfg<T>::PFM>), dim3(row_grid), dim3(NT), lds, st, row,
+                           post, in, in_bstride, rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows, batch,
+                           vec);
+        return hipGetLastError();
+    }
     const dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
     hipLaunchKernelGGL((k_row_c2r<T, NT, VMAX, RB>), grid, dim3(NT), lds, st, row, post,
-                       in, in_bstride, rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows);
+                       in, in_bstride, rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows, vec);
     return hipGetLastError();
 }
 

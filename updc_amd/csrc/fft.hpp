@@ -32,6 +32,8 @@ template <class T> struct Fft2 {
     Cx<T> *tw_row = nullptr, *tw_col = nullptr, *post = nullptr;  // device tables
     int col_re = 0;          // edge radix of the column plan (0: LDS-only column kernel)
     bool allow_edge = true;  // set before init(): false forces the LDS-only column kernel
+    bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit
+    int row_grid = 0;            // workgroups of the persistent row passes (one per CU)
 
     // Returns "" on success or an error message.
     std::string init(int FW, int FH);
