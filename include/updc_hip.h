@@ -144,6 +144,19 @@ int updc_fft_shape_candidates(int min_width, int min_height, int *widths, int *h
  * fft_autotune_v1_*.txt in the cache root, updc_core.c:181-223); "" or NULL
  * disables it.  retune != 0 ignores a cached result (updc_core.c:198). */
 int updc_ctx_set_fft_cache(updc_ctx *ctx, const char *dir, int retune);
+/* Persistent set-up caches of the reference (updc_core.c:145-158, 790-843,
+ * 905-976, 1024-1083) in directory `dir` ("" or NULL: none), for the F32
+ * tier (the reference's arithmetic; the other tiers never read or write
+ * them): psf_spectrum_%02d_%dx%d_t%d.c64 (FH x (FW/2+1) complex64, row-major,
+ * FFTW's R2C layout), initial_estimate.f32 and initial_coverage.f32 (H x W),
+ * denominator_%dx%d.f32 (H x W).  A file is used when its size is exact,
+ * else recomputed and written (temporary file + rename).  Set before
+ * updc_prepare. */
+int updc_ctx_set_setup_cache(updc_ctx *ctx, const char *dir);
+/* Caches the last updc_prepare read: bit 0 PSF spectra, bit 1 initial
+ * estimate and coverage, bit 2 denominator. */
+int updc_get_setup_cache_hits(updc_ctx *ctx, int *hits);
+
 /* Column kernel of the 2-D FFT: -1 automatic (edge-stage where the column
  * length allows), 0 LDS-only, 1 edge-stage where the length allows.  Set
  * before updc_prepare; no reference counterpart (a tile choice). */

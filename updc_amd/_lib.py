@@ -57,6 +57,8 @@ SIGNATURES = {
     "updc_fft_shape_candidates": (C.c_int, [C.c_int, C.c_int, _I, _I, C.c_int, _I]),
     "updc_ctx_set_fft_cache": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "updc_ctx_set_column_kernel": (C.c_int, [_P, C.c_int]),
+    "updc_ctx_set_setup_cache": (C.c_int, [_P, C.c_char_p]),
+    "updc_get_setup_cache_hits": (C.c_int, [_P, _I]),
     "updc_get_autotune": (C.c_int, [_P, _I, _I, _I, _D, C.c_int, _I, _I]),
     "updc_op_layout": (C.c_int, [_P, _I, _I, _I, _I, _I]),
     "updc_op_r2c": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),

@@ -193,6 +193,7 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
     check(updc_ctx_create(&g.c, W, H, 0, 0, prec, flags, env_int("UPDC_HIP_DEVICE", 0)));
     check(updc_ctx_set_preprocess_workers(g.c, prew));
     if (autotune && persist) check(updc_ctx_set_fft_cache(g.c, cache_root(manifest).c_str(), retune));
+    if (persist) check(updc_ctx_set_setup_cache(g.c, cache_root(manifest).c_str()));
 
     double t0 = now();
     double t_links = 0, t_data = 0;

@@ -239,6 +239,29 @@ struct PipeBase {
     std::vector<int> stage_n;
 };
 
+// ------------------------------------------------------------------ set-up caches
+// The reference's persistent set-up caches (updc_core.c:145-158, 790-843,
+// 905-976, 1024-1083): raw little-endian files in the cache root, used only
+// when their size is exact, written through a temporary file and a rename.
+static bool load_exact(const std::string &path, void *dst, size_t bytes) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    bool ok = std::fseek(f, 0, SEEK_END) == 0 && (size_t)std::ftell(f) == bytes && std::fseek(f, 0, SEEK_SET) == 0 &&
+              std::fread(dst, 1, bytes, f) == bytes;
+    std::fclose(f);
+    return ok;
+}
+static void save_atomic(const std::string &path, const void *src, size_t bytes) {
+    const std::string tmp = path + ".tmp." + std::to_string((long)getpid());
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) throw Error(UPDC_ERR_IO, "cannot create cache file " + tmp);
+    const bool ok = std::fwrite(src, 1, bytes, f) == bytes;
+    if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        throw Error(UPDC_ERR_IO, "cache write failed: " + path);
+    }
+}
+
 struct TuneRow {
     int w, h, edge;  // shape and column kernel (1 edge-stage, 0 LDS-only)
     double ms;       // batched round trip; < 0 when the shape is unsupported
@@ -253,6 +276,8 @@ struct Ctx {
     bool fft_retune = false;     // ignore a cached autotune result
     std::vector<TuneRow> tune;   // the last autotune's measurements
     bool tune_cached = false;
+    std::string setup_cache_dir; // PSF spectrum / initial estimate / denominator caches ("" = none)
+    int setup_cache_hits = 0;    // caches read by the last prepare (bit 0 PSF spectra, 1 initial, 2 D)
     std::vector<std::unique_ptr<HostExp>> ex;
     hipStream_t st = nullptr;
     bool prepared = false;
@@ -444,25 +469,96 @@ template <class TF, class TA> struct Pipe : PipeBase {
         build_csc(c);
         upload_ex(c);
 
-        // PSF spectra
+        // PSF spectra (cached for the F32 tier in the reference's FFTW layout:
+        // FH x (FW/2+1) complex64, row-major)
+        const bool cache = !c.setup_cache_dir.empty() && c.prec == UPDC_PREC_F32;
+        c.setup_cache_hits = 0;
         psf_spec.alloc((size_t)n * fft.spec_elems(s.FH));
-        psf_spectra<TF>(c, fft, psf_spec.p);
+        {
+            const size_t P = rp(s.FH), one = fft.spec_elems(s.FH), nc = (size_t)s.FH * s.NC;
+            std::vector<std::string> paths(n);
+            std::vector<Cx<float>> file, cm;
+            bool all = cache;
+            for (int k = 0; k < n; ++k) {
+                char nm[96];
+                std::snprintf(nm, sizeof nm, "/psf_spectrum_%02d_%dx%d_t%d.c64", k + 1, s.FW, s.FH, c.ex[k]->psf_t);
+                paths[k] = c.setup_cache_dir + nm;
+            }
+            if (all) {
+                file.resize(nc);
+                cm.assign(one, Cx<float>{0.f, 0.f});
+                for (int k = 0; k < n && all; ++k) {
+                    if (!load_exact(paths[k], file.data(), nc * sizeof(Cx<float>))) { all = false; break; }
+                    for (int r = 0; r < s.FH; ++r)
+                        for (int v = 0; v < s.NC; ++v) cm[(size_t)v * P + r] = file[(size_t)r * s.NC + v];
+                    HIPCHK(hipMemcpy(psf_spec.p + (size_t)k * one, cm.data(), one * sizeof(Cx<float>),
+                                     hipMemcpyHostToDevice));
+                }
+            }
+            if (all) {
+                c.setup_cache_hits |= 1;
+            } else {
+                psf_spectra<TF>(c, fft, psf_spec.p);
+                if (cache) {
+                    HIPCHK(hipStreamSynchronize(c.st));
+                    file.resize(nc);
+                    cm.resize(one);
+                    for (int k = 0; k < n; ++k) {
+                        HIPCHK(hipMemcpy(cm.data(), psf_spec.p + (size_t)k * one, one * sizeof(Cx<float>),
+                                         hipMemcpyDeviceToHost));
+                        for (int r = 0; r < s.FH; ++r)
+                            for (int v = 0; v < s.NC; ++v) file[(size_t)r * s.NC + v] = cm[(size_t)v * P + r];
+                        save_atomic(paths[k], file.data(), nc * sizeof(Cx<float>));
+                    }
+                }
+            }
+        }
 
         // initial drizzle (updc_core.c:905-983)
         x.alloc(nout);
         cov.alloc(nout);
-        HIPCHK(fill<unsigned int>(err.p, 0u, 1, c.st));
-        HIPCHK(drizzle<TF>(d_ex.p, n, sci.p, w.p, std::max(1, std::min(c.workers, n)), nout, x.p, cov.p, err.p, c.st));
-        HIPCHK(hipMemcpyAsync(&ebits, err.p, 4, hipMemcpyDeviceToHost, c.st));
-        HIPCHK(hipStreamSynchronize(c.st));
-        if (ebits & UPDC_E_DRIZZLE) throw Error(UPDC_ERR_NUMERIC, "non-finite signed initial estimate");
+        {
+            const std::string pe = c.setup_cache_dir + "/initial_estimate.f32";
+            const std::string pc = c.setup_cache_dir + "/initial_coverage.f32";
+            std::vector<float> he(cache ? nout : 0), hc(cache ? nout : 0);
+            if (cache && load_exact(pe, he.data(), nout * 4) && load_exact(pc, hc.data(), nout * 4)) {
+                HIPCHK(hipMemcpy(x.p, he.data(), nout * 4, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(cov.p, hc.data(), nout * 4, hipMemcpyHostToDevice));
+                c.setup_cache_hits |= 2;
+            } else {
+                HIPCHK(fill<unsigned int>(err.p, 0u, 1, c.st));
+                HIPCHK(drizzle<TF>(d_ex.p, n, sci.p, w.p, std::max(1, std::min(c.workers, n)), nout, x.p, cov.p,
+                                   err.p, c.st));
+                HIPCHK(hipMemcpyAsync(&ebits, err.p, 4, hipMemcpyDeviceToHost, c.st));
+                HIPCHK(hipStreamSynchronize(c.st));
+                if (ebits & UPDC_E_DRIZZLE) throw Error(UPDC_ERR_NUMERIC, "non-finite signed initial estimate");
+                if (cache) {
+                    HIPCHK(hipMemcpy(he.data(), x.p, nout * 4, hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(hc.data(), cov.p, nout * 4, hipMemcpyDeviceToHost));
+                    save_atomic(pe, he.data(), nout * 4);
+                    save_atomic(pc, hc.data(), nout * 4);
+                }
+            }
+        }
         x_init.alloc(nout);
         HIPCHK(hipMemcpyAsync(x_init.p, x.p, nout * sizeof(TF), hipMemcpyDeviceToDevice, c.st));
 
         // denominator
         D.alloc(nout);
-        if (sizeof(TA) == sizeof(TF)) {
+        char dname[64];
+        std::snprintf(dname, sizeof dname, "/denominator_%dx%d.f32", s.FW, s.FH);
+        const std::string pd = c.setup_cache_dir + dname;
+        std::vector<float> hd(cache ? nout : 0);
+        if (cache && load_exact(pd, hd.data(), nout * 4)) {
+            HIPCHK(hipMemcpy(D.p, hd.data(), nout * 4, hipMemcpyHostToDevice));
+            c.setup_cache_hits |= 4;
+        } else if (sizeof(TA) == sizeof(TF)) {
             denominator<TF>(c, fft, psf_spec.p, d_ex.p, w.p, (TF *)D.p);
+            if (cache) {
+                HIPCHK(hipStreamSynchronize(c.st));
+                HIPCHK(hipMemcpy(hd.data(), D.p, nout * 4, hipMemcpyDeviceToHost));
+                save_atomic(pd, hd.data(), nout * 4);
+            }
         } else {
             // mixed tier: the denominator is computed entirely in FP64
             Fft2<double> f64;  // edge-stage columns where possible: the FP64 LDS-only kernel stops at FH 6144
@@ -1354,6 +1450,20 @@ int updc_ctx_set_fft_cache(updc_ctx *ctx, const char *dir, int retune) {
         need(!ctx->prepared, UPDC_ERR_STATE, "context already prepared");
         ctx->fft_cache_dir = dir ? dir : "";
         ctx->fft_retune = retune != 0;
+    });
+}
+
+int updc_ctx_set_setup_cache(updc_ctx *ctx, const char *dir) {
+    return guarded([&] {
+        need(ctx != nullptr, UPDC_ERR_ARG, "null context");
+        need(!ctx->prepared, UPDC_ERR_STATE, "context already prepared");
+        ctx->setup_cache_dir = dir ? dir : "";
+    });
+}
+int updc_get_setup_cache_hits(updc_ctx *ctx, int *hits) {
+    return guarded([&] {
+        need(ctx != nullptr && hits, UPDC_ERR_ARG, "null argument");
+        *hits = ctx->setup_cache_hits;
     });
 }
 

@@ -54,7 +54,7 @@ class Context:
     def __init__(self, width: int, height: int, precision: str = "f32", fft_shape: tuple[int, int] | None = None,
                  graph: bool = False, quiet: bool = True, device: int = 0, preprocess_workers: int = 1,
                  autotune: bool = False, fft_cache_dir: str | None = None, retune: bool = False,
-                 column_kernel: int = -1):
+                 column_kernel: int = -1, setup_cache_dir: str | None = None):
         self.lib = _lib.load()
         self.W, self.H = int(width), int(height)
         self.precision = precision
@@ -76,6 +76,8 @@ class Context:
             check(self.lib.updc_ctx_set_fft_cache(self.h, (fft_cache_dir or "").encode(), int(retune)))
         if column_kernel != -1:
             check(self.lib.updc_ctx_set_column_kernel(self.h, int(column_kernel)))
+        if setup_cache_dir is not None:
+            check(self.lib.updc_ctx_set_setup_cache(self.h, setup_cache_dir.encode()))
 
     # -- lifetime
     def close(self) -> None:
@@ -99,6 +101,12 @@ class Context:
         """Plain form: one launch per exposure and stage (no batching)."""
         self.flags = (self.flags | FLAG_PER_EXPOSURE) if on else (self.flags & ~FLAG_PER_EXPOSURE)
         check(self.lib.updc_ctx_set_flags(self.h, self.flags))
+
+    def setup_cache_hits(self) -> dict:
+        """Which set-up caches the last prepare read."""
+        h = C.c_int()
+        check(self.lib.updc_get_setup_cache_hits(self.h, C.byref(h)))
+        return dict(psf_spectra=bool(h.value & 1), initial=bool(h.value & 2), denominator=bool(h.value & 4))
 
     def autotune_report(self) -> tuple[list[tuple[int, int, int, float]], bool]:
         """(width, height, column_edge, ms) per timed candidate of the last
