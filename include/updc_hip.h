@@ -230,6 +230,22 @@ int updc_stage_scatter(updc_ctx *ctx, int k, const double *v, double *out);
 /* Zero every FreezeMu mask. */
 int updc_reset_frozen(updc_ctx *ctx);
 
+/* ---- PSF projector ----------------------------------------------------------
+ * Polygon push-forward of a PSF onto the target grid (psf_projector.c:82-172):
+ * input pixel (x, y) of the width x height PSF maps to the quad of vertices
+ * (y, x), (y+1, x), (y+1, x+1), (y, x+1) of the (height+1) x (width+1) x 2
+ * lattice (target-pixel offsets from the PSF centre); every overlap area
+ * > 1e-8 adds value * area / full_area to the output pixel, in ascending input
+ * order (fmaf).  Output: out_width x out_height = (2 hx + 1) x (2 hy + 1),
+ * hx = ceil(max|x| - 0.5 - 1e-7) (0 if negative), row-major, on the GPU of
+ * `device`.  out = NULL queries the shape.  Errors as the reference's:
+ * negative/non-finite PSF, zero input sum, non-finite vertex, |output_sum /
+ * input_sum - 1| > 5e-5 (UPDC_ERR_NUMERIC; the output and sums are still
+ * written).  overlaps = the number of links. */
+int updc_psf_project(int device, const float *psf, int width, int height, const double *vertices, float *out,
+                     size_t out_capacity, int *out_width, int *out_height, double *input_sum, double *output_sum,
+                     uint64_t *overlaps);
+
 /* ---- device-pointer operators (the autograd wrapper) ------------------------
  * On a prepared context, with DEVICE pointers of the context's element type
  * (float for UPDC_PREC_F32 and UPDC_PREC_MIXED, double for UPDC_PREC_F64),

@@ -26,6 +26,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 REF_LIB = HERE / "_ref" / "liboverlap_ref.so"
+REF_PROJECTOR = HERE / "_ref" / "updc_psf_projector"
 
 _lib = None
 _ref = None
@@ -66,6 +67,8 @@ def lib() -> C.CDLL:
             "orc_accumulate_adjoint": (None, [F, F, F, C.c_size_t]),
             "orc_crop_scale": (None, [F, C.c_int, C.c_int, C.c_int, C.c_int, F]),
             "orc_update": (C.c_int, [F, F, F, C.c_size_t, C.c_float, F, U64, U64]),
+            "orc_psf_extent": (C.c_int, [D, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "orc_psf_project": (C.c_uint64, [F, C.c_int, C.c_int, D, C.c_int, C.c_int, F]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -111,6 +114,23 @@ def fft_shape_list(min_w: int, min_h: int) -> list[tuple[int, int]]:
     out = (C.c_int * 32)()
     n = lib().orc_fft_shape_list(int(min_w), int(min_h), out)
     return [(out[2 * i], out[2 * i + 1]) for i in range(n)]
+
+
+def psf_project(psf: np.ndarray, vertices: np.ndarray) -> tuple[np.ndarray, int]:
+    """Polygon push-forward of a PSF (psf_projector.c:71-146): the projected
+    (2hy+1) x (2hx+1) float32 PSF and the overlap count."""
+    p = np.ascontiguousarray(psf, np.float32)
+    v = np.ascontiguousarray(vertices, np.float64)
+    ih, iw = p.shape
+    if v.shape != (ih + 1, iw + 1, 2):
+        raise ValueError("vertex lattice must be (ih+1, iw+1, 2)")
+    L = lib()
+    hx, hy = C.c_int(), C.c_int()
+    if L.orc_psf_extent(_p(v, C.c_double), iw, ih, C.byref(hx), C.byref(hy)):
+        raise FloatingPointError("non-finite PSF mapped vertex")
+    out = np.empty((2 * hy.value + 1, 2 * hx.value + 1), np.float32)
+    n = L.orc_psf_project(_p(p, C.c_float), iw, ih, _p(v, C.c_double), hx.value, hy.value, _p(out, C.c_float))
+    return out, int(n)
 
 
 def build_linkmap(vertices: np.ndarray, W: int, H: int) -> bytes:

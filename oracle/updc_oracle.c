@@ -619,3 +619,73 @@ int orc_update(float *x, const float *C, const float *D, size_t nout, float max_
     *nneg_out = nneg;
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* PSF polygon push-forward (psf_projector.c:71-146).                 */
+/* Pinned against the reference's own psf_projector.c built into      */
+/* oracle/_ref (tests/test_oracle_psf_projector.py), plus the identity */
+/* mapping and flux conservation.                                      */
+/* ------------------------------------------------------------------ */
+static int orc_half_extent(double max_abs) {
+    /* psf_projector.c:71-80 */
+    double needed = max_abs - 0.5 - 1.0e-7;
+    if (needed <= 0.0) return 0;
+    return (int)ceil(needed);
+}
+
+/* Output half extents from the vertex lattice ((ih+1) x (iw+1) x 2). */
+int orc_psf_extent(const double *v, int iw, int ih, int *hx, int *hy) {
+    double mx = 0.0, my = 0.0;
+    const size_t n = (size_t)(iw + 1) * (size_t)(ih + 1);
+    for (size_t i = 0; i < n; ++i) {
+        const double x = v[2 * i], y = v[2 * i + 1];
+        if (!isfinite(x) || !isfinite(y)) return -1;
+        if (fabs(x) > mx) mx = fabs(x);
+        if (fabs(y) > my) my = fabs(y);
+    }
+    *hx = orc_half_extent(mx);
+    *hy = orc_half_extent(my);
+    return 0;
+}
+
+/* out ((2hy+1) x (2hx+1), zeroed here) += psf pushed through the quads,
+ * psf_projector.c:107-146; returns the overlap count. */
+uint64_t orc_psf_project(const float *psf, int iw, int ih, const double *v, int hx, int hy, float *out) {
+    const int ow = 2 * hx + 1, oh = 2 * hy + 1;
+    memset(out, 0, (size_t)ow * (size_t)oh * sizeof(float));
+    uint64_t overlaps = 0;
+    for (int y = 0; y < ih; ++y)
+        for (int x = 0; x < iw; ++x) {
+            const float value = psf[(size_t)y * iw + x];
+            if (value == 0.0f) continue;
+            const size_t a = 2 * ((size_t)y * (size_t)(iw + 1) + (size_t)x);
+            const size_t b = a + 2 * (size_t)(iw + 1);
+            OrcPt q[4] = {{v[a], v[a + 1]}, {v[b], v[b + 1]}, {v[b + 2], v[b + 3]}, {v[a + 2], v[a + 3]}};
+            const float full = orc_poly_area(q, 4);
+            if (!isfinite(full) || !(full > 1.0e-12f)) continue;
+            double mnx = q[0].x, mxx = q[0].x, mny = q[0].y, mxy = q[0].y;
+            for (int j = 1; j < 4; ++j) {
+                if (q[j].x < mnx) mnx = q[j].x;
+                if (q[j].x > mxx) mxx = q[j].x;
+                if (q[j].y < mny) mny = q[j].y;
+                if (q[j].y > mxy) mxy = q[j].y;
+            }
+            int x0 = (int)floor(mnx + 0.5), x1 = (int)floor(mxx + 0.5);
+            int y0 = (int)floor(mny + 0.5), y1 = (int)floor(mxy + 0.5);
+            if (x0 < -hx) x0 = -hx;
+            if (x1 > hx) x1 = hx;
+            if (y0 < -hy) y0 = -hy;
+            if (y1 > hy) y1 = hy;
+            for (int ty = y0; ty <= y1; ++ty)
+                for (int tx = x0; tx <= x1; ++tx) {
+                    const float area = orc_quad_overlap(q, tx, ty);
+                    if (!(area > 1.0e-8f)) continue;
+                    const float frac = area / full;
+                    const size_t o = (size_t)(ty + hy) * ow + (size_t)(tx + hx);
+                    out[o] = fmaf(value, frac, out[o]);
+                    ++overlaps;
+                }
+        }
+    (void)oh;
+    return overlaps;
+}

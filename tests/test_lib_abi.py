@@ -41,3 +41,12 @@ def test_dropin_executable_self_test():
     assert r.returncode == 0 and "UPDC C core self-test: OK" in r.stdout
     bad = subprocess.run([str(_lib.EXE_PATH), "a", "b"], capture_output=True, text=True)
     assert bad.returncode == 1 and bad.stderr.startswith("UPDC_ALL: usage")
+
+
+def test_dropin_psf_projector_self_test():
+    import subprocess
+    from updc_amd import _lib
+    r = subprocess.run([str(_lib.PROJECTOR_PATH), "--self-test"], capture_output=True, text=True)
+    assert r.returncode == 0 and "UPDC PSF projector self-test: OK" in r.stdout
+    bad = subprocess.run([str(_lib.PROJECTOR_PATH), "a"], capture_output=True, text=True)
+    assert bad.returncode == 1 and bad.stderr.startswith("UPDC_PSF_PROJECT: usage")

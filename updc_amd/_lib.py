@@ -11,6 +11,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libupdc_hip.so"
 EXE_PATH = LIB_PATH.parent / "updc_hip_core"
+PROJECTOR_PATH = LIB_PATH.parent / "updc_hip_psf_projector"
 
 UPDC_OK = 0
 ERR_NAMES = {1: "ARG", 2: "HIP", 3: "IO", 4: "NUMERIC", 5: "STATE", 6: "NOMEM", 7: "FORMAT"}
@@ -60,6 +61,8 @@ SIGNATURES = {
     "updc_ctx_set_setup_cache": (C.c_int, [_P, C.c_char_p]),
     "updc_get_setup_cache_hits": (C.c_int, [_P, _I]),
     "updc_get_autotune": (C.c_int, [_P, _I, _I, _I, _D, C.c_int, _I, _I]),
+    "updc_psf_project": (C.c_int, [C.c_int, _F, C.c_int, C.c_int, _D, _F, C.c_size_t, _I, _I, _D, _D,
+                                   C.POINTER(C.c_uint64)]),
     "updc_op_layout": (C.c_int, [_P, _I, _I, _I, _I, _I]),
     "updc_op_r2c": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "updc_op_c2r": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, _P]),

@@ -5,6 +5,7 @@
 Outputs (git-ignored, shipped with the tree):
     updc_amd/lib/libupdc_hip.so   -- C-ABI library (include/updc_hip.h)
     updc_amd/lib/updc_hip_core    -- drop-in for the reference's updc_jwst_c
+    updc_amd/lib/updc_hip_psf_projector -- drop-in for the reference's updc_psf_projector
 """
 from __future__ import annotations
 
@@ -19,6 +20,7 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libupdc_hip.so"
 EXE = LIBDIR / "updc_hip_core"
+PROJECTOR = LIBDIR / "updc_hip_psf_projector"
 
 HIP_SOURCES = ["fft.hip", "stages.hip", "solver.hip", "driver.hip"]
 ARCH = os.environ.get("UPDC_OFFLOAD_ARCH", "gfx950")
@@ -71,8 +73,9 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB)
-    _run([_hipcc(), *common_flags(), "-o", str(EXE), str(CSRC / "updc_hip_core.cpp"),
-          f"-L{LIBDIR}", "-lupdc_hip", f"-Wl,-rpath,$ORIGIN"])
+    for exe, src in ((EXE, "updc_hip_core.cpp"), (PROJECTOR, "psf_projector_main.cpp")):
+        _run([_hipcc(), *common_flags(), "-o", str(exe), str(CSRC / src), f"-L{LIBDIR}", "-lupdc_hip",
+              f"-Wl,-rpath,$ORIGIN"])
     for o in objs:
         o.unlink(missing_ok=True)
     return LIB
@@ -80,4 +83,4 @@ def build(verbose: bool = False) -> Path:
 
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
-    print(f"built {LIB} and {EXE}")
+    print(f"built {LIB}, {EXE} and {PROJECTOR}")

@@ -1453,6 +1453,127 @@ int updc_ctx_set_fft_cache(updc_ctx *ctx, const char *dir, int retune) {
     });
 }
 
+// ---- PSF projector (psf_projector.c:82-172)
+// Half extent of the output grid: the smallest h with max |coordinate| <= h +
+// 0.5 (+1e-7 tolerance), psf_projector.c:71-80.
+static int pp_half_extent(double max_abs) {
+    if (!(max_abs >= 0.0) || !std::isfinite(max_abs)) throw Error(UPDC_ERR_NUMERIC, "invalid PSF mapped extent");
+    const double needed = max_abs - 0.5 - 1.0e-7;
+    if (needed <= 0.0) return 0;
+    if (needed > 1000000.0) throw Error(UPDC_ERR_ARG, "unreasonable PSF mapped extent");
+    return (int)std::ceil(needed);
+}
+int updc_psf_project(int device, const float *psf, int width, int height, const double *vertices, float *out,
+                     size_t out_capacity, int *out_width, int *out_height, double *input_sum, double *output_sum,
+                     uint64_t *overlaps) {
+    return guarded([&] {
+        need(psf && vertices && out_width && out_height && width >= 1 && height >= 1, UPDC_ERR_ARG,
+             "invalid PSF projection arguments");
+        const size_t nin = (size_t)width * height, nvert = (size_t)(width + 1) * (height + 1);
+        double isum = 0.0;
+        for (size_t i = 0; i < nin; ++i) {
+            if (!std::isfinite(psf[i]) || psf[i] < 0.0f)
+                throw Error(UPDC_ERR_NUMERIC, "PSF contains negative/non-finite pixels");
+            isum += (double)psf[i];
+        }
+        if (!(isum > 0.0)) throw Error(UPDC_ERR_NUMERIC, "PSF input sum is zero");
+        double mx = 0.0, my = 0.0;
+        for (size_t i = 0; i < nvert; ++i) {
+            const double x = vertices[2 * i], y = vertices[2 * i + 1];
+            if (!std::isfinite(x) || !std::isfinite(y)) throw Error(UPDC_ERR_NUMERIC, "non-finite PSF mapped vertex");
+            mx = std::max(mx, std::fabs(x));
+            my = std::max(my, std::fabs(y));
+        }
+        const int hx = pp_half_extent(mx), hy = pp_half_extent(my);
+        const int ow = 2 * hx + 1, oh = 2 * hy + 1;
+        const size_t nout = (size_t)ow * oh;
+        *out_width = ow;
+        *out_height = oh;
+        if (!out) return;  // size query
+        need(out_capacity >= nout, UPDC_ERR_ARG, "PSF projection output buffer too small");
+        HIPCHK(hipSetDevice(device));
+        hipStream_t st = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        try {
+            DevBuf<double> dv;
+            DevBuf<float> dval, dfrac, dout;
+            DevBuf<uint32_t> cnt, off, t32, tcnt, toff, cursor;
+            DevBuf<uint2> tent;
+            DevBuf<uint8_t> scratch;
+            DevBuf<unsigned long long> tot;
+            DevBuf<unsigned int> flags;
+            DevBuf<ExpDev> one;
+            dv.alloc(2 * nvert);
+            dval.alloc(nin);
+            HIPCHK(hipMemcpyAsync(dv.p, vertices, 2 * nvert * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dval.p, psf, nin * sizeof(float), hipMemcpyHostToDevice, st));
+            cnt.alloc(nin);
+            off.alloc(nin + 1);
+            scratch.alloc(scan_scratch_bytes(std::max(nin, nout)));
+            tot.alloc(1);
+            flags.alloc(1);
+            HIPCHK(fill<unsigned int>(flags.p, 0u, 1, st));
+            HIPCHK(pp_count(dv.p, dval.p, width, height, hx, hy, cnt.p, st));
+            HIPCHK(scan_u32(cnt.p, off.p, nin, scratch.p, tot.p, st));
+            unsigned long long nl = 0;
+            HIPCHK(hipMemcpyAsync(&nl, tot.p, sizeof nl, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (nl > 0xffffffffull) throw Error(UPDC_ERR_ARG, "PSF projection exceeds uint32 link capacity");
+            t32.alloc(nl);
+            dfrac.alloc(nl);
+            HIPCHK(pp_fill(dv.p, dval.p, width, height, hx, hy, off.p, t32.p, dfrac.p, flags.p, st));
+            // transposed (per output pixel, ascending input pixel) list, then the ordered sums
+            ExpDev e{};
+            e.nin = (uint32_t)nin;
+            e.encoding = UPDC_LM_ABS32;
+            e.off = off.p;
+            e.t32 = t32.p;
+            e.frac = dfrac.p;
+            e.pix = 0;
+            one.alloc(1);
+            HIPCHK(hipMemcpyAsync(one.p, &e, sizeof e, hipMemcpyHostToDevice, st));
+            tcnt.alloc(nout);
+            toff.alloc(nout + 1);
+            HIPCHK(fill<uint32_t>(tcnt.p, 0u, nout, st));
+            HIPCHK(csc_count<float>(one.p, (uint32_t)nin, dval.p, tcnt.p, st));
+            HIPCHK(scan_u32(tcnt.p, toff.p, nout, scratch.p, tot.p, st));
+            unsigned long long ne = 0;
+            HIPCHK(hipMemcpyAsync(&ne, tot.p, sizeof ne, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            tent.alloc(ne);
+            cursor.alloc(nout + 1);
+            HIPCHK(hipMemcpyAsync(cursor.p, toff.p, nout * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+            HIPCHK(csc_fill<float>(one.p, (uint32_t)nin, dval.p, cursor.p, tent.p, st));
+            HIPCHK(csc_sort(toff.p, tent.p, nout, st));
+            e.toff = toff.p;
+            e.tent = tent.p;
+            HIPCHK(hipMemcpyAsync(one.p, &e, sizeof e, hipMemcpyHostToDevice, st));
+            dout.alloc(nout);
+            HIPCHK(csc_apply<float>(one.p, dval.p, nout, dout.p, st));
+            unsigned int fl = 0;
+            HIPCHK(hipMemcpyAsync(&fl, flags.p, 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(out, dout.p, nout * sizeof(float), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (fl) throw Error(UPDC_ERR_FORMAT, "PSF projection link fill mismatch");
+            double osum = 0.0;
+            for (size_t i = 0; i < nout; ++i) {
+                if (!std::isfinite(out[i]) || out[i] < 0.0f)
+                    throw Error(UPDC_ERR_NUMERIC, "PSF contains negative/non-finite pixels");
+                osum += (double)out[i];
+            }
+            if (input_sum) *input_sum = isum;
+            if (output_sum) *output_sum = osum;
+            if (overlaps) *overlaps = nl;
+            if (std::fabs((osum - isum) / isum) > 5.0e-5)
+                throw Error(UPDC_ERR_NUMERIC, "PSF polygon projection is not flux-conserving");
+        } catch (...) {
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        HIPCHK(hipStreamDestroy(st));
+    });
+}
+
 int updc_ctx_set_setup_cache(updc_ctx *ctx, const char *dir) {
     return guarded([&] {
         need(ctx != nullptr, UPDC_ERR_ARG, "null context");

@@ -186,6 +186,92 @@ hipError_t lm_fill(const double *vert, int iw, int ih, int W, int H, const uint3
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- PSF projector
+// The PSF push-forward of psf_projector.c:82-146: input pixel (x, y) with a
+// non-zero value and mapped area > 1e-12 links to every output pixel (tx, ty)
+// of its clamped bounding box in [-hx, hx] x [-hy, hy] whose overlap area
+// exceeds 1e-8, with frac = area / full_area; output index
+// (ty + hy) * (2 hx + 1) + (tx + hx).  Two passes (count, then fill) into an
+// ABS32 link list in (ty, tx) order per pixel; the weighted sum per output
+// pixel then runs over the transposed list in ascending input pixel order,
+// the reference's fmaf(value, frac, out) sequence.
+__device__ bool pp_pixel(const double *__restrict__ v, const float *__restrict__ val, int iw, int hx, int hy, int x,
+                         int y, DPt q[4], float &full, int &x0, int &x1, int &y0, int &y1) {
+    if (val[(size_t)y * iw + x] == 0.0f) return false;
+    const size_t a = 2 * ((size_t)y * (size_t)(iw + 1) + (size_t)x), b = a + 2 * (size_t)(iw + 1);
+    q[0] = {v[a], v[a + 1]};
+    q[1] = {v[b], v[b + 1]};
+    q[2] = {v[b + 2], v[b + 3]};
+    q[3] = {v[a + 2], v[a + 3]};
+    full = g_area(q, 4);
+    if (!isfinite(full) || !(full > 1.0e-12f)) return false;
+    double mnx = q[0].x, mxx = mnx, mny = q[0].y, mxy = mny;
+    for (int j = 1; j < 4; ++j) {
+        mnx = q[j].x < mnx ? q[j].x : mnx;
+        mxx = q[j].x > mxx ? q[j].x : mxx;
+        mny = q[j].y < mny ? q[j].y : mny;
+        mxy = q[j].y > mxy ? q[j].y : mxy;
+    }
+    x0 = (int)floor(mnx + .5); x1 = (int)floor(mxx + .5);
+    y0 = (int)floor(mny + .5); y1 = (int)floor(mxy + .5);
+    if (x0 < -hx) x0 = -hx;
+    if (x1 > hx) x1 = hx;
+    if (y0 < -hy) y0 = -hy;
+    if (y1 > hy) y1 = hy;
+    return true;
+}
+__global__ __launch_bounds__(256) void k_pp_count(const double *__restrict__ vert, const float *__restrict__ val, int iw,
+                                                  int ih, int hx, int hy, uint32_t *__restrict__ cnt) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (size_t)iw * ih) return;
+    DPt q[4];
+    float full;
+    int x0, x1, y0, y1;
+    uint32_t n = 0;
+    if (pp_pixel(vert, val, iw, hx, hy, (int)(p % iw), (int)(p / iw), q, full, x0, x1, y0, y1))
+        for (int ty = y0; ty <= y1; ++ty)
+            for (int tx = x0; tx <= x1; ++tx)
+                if (g_overlap(q, tx, ty) > 1.0e-8f) ++n;
+    cnt[p] = n;
+}
+__global__ __launch_bounds__(256) void k_pp_fill(const double *__restrict__ vert, const float *__restrict__ val, int iw,
+                                                 int ih, int hx, int hy, const uint32_t *__restrict__ off,
+                                                 uint32_t *__restrict__ t32, float *__restrict__ frac,
+                                                 unsigned int *flags) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (size_t)iw * ih) return;
+    DPt q[4];
+    float full;
+    int x0, x1, y0, y1;
+    uint32_t z = off[p];
+    const uint32_t zend = off[p + 1];
+    const int ow = 2 * hx + 1;
+    if (pp_pixel(vert, val, iw, hx, hy, (int)(p % iw), (int)(p / iw), q, full, x0, x1, y0, y1))
+        for (int ty = y0; ty <= y1; ++ty)
+            for (int tx = x0; tx <= x1; ++tx) {
+                const float area = g_overlap(q, tx, ty);
+                if (!(area > 1.0e-8f)) continue;
+                if (z >= zend) { atomicOr(flags, 2u); return; }
+                t32[z] = (uint32_t)((ty + hy) * ow + (tx + hx));
+                frac[z] = area / full;
+                ++z;
+            }
+    if (z != zend) atomicOr(flags, 8u);
+}
+hipError_t pp_count(const double *vert, const float *val, int iw, int ih, int hx, int hy, uint32_t *cnt,
+                    hipStream_t st) {
+    const size_t n = (size_t)iw * ih;
+    hipLaunchKernelGGL(k_pp_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vert, val, iw, ih, hx, hy, cnt);
+    return hipGetLastError();
+}
+hipError_t pp_fill(const double *vert, const float *val, int iw, int ih, int hx, int hy, const uint32_t *off,
+                   uint32_t *t32, float *frac, unsigned int *flags, hipStream_t st) {
+    const size_t n = (size_t)iw * ih;
+    hipLaunchKernelGGL(k_pp_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vert, val, iw, ih, hx, hy, off,
+                       t32, frac, flags);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_lm_validate(const uint32_t *__restrict__ off, const uint32_t *__restrict__ base,
                                                      const uint16_t *__restrict__ d16, const uint32_t *__restrict__ t32,
                                                      const float *__restrict__ frac, uint32_t encoding, uint32_t nin,

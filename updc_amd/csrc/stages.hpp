@@ -7,6 +7,11 @@ namespace updc {
 // ---- geometry (device restatement of overlap.c, bit-exact) ----------------
 // Pass 1 of the LinkMap build (updc_core.c:549-568): per-pixel link count,
 // first target, and flags bit 0 when some stencil spans more than 65535.
+// PSF projector link lists (psf_projector.c:82-146), see stages.hip.
+hipError_t pp_count(const double *vert, const float *val, int iw, int ih, int hx, int hy, uint32_t *cnt,
+                    hipStream_t st);
+hipError_t pp_fill(const double *vert, const float *val, int iw, int ih, int hx, int hy, const uint32_t *off,
+                   uint32_t *t32, float *frac, unsigned int *flags, hipStream_t st);
 hipError_t lm_count(const double *vert, int iw, int ih, int W, int H, uint32_t *cnt, uint32_t *first,
                     unsigned int *flags, hipStream_t st);
 // Pass 2 (updc_core.c:586-597): fill delta16/target32 and frac at off[p].
