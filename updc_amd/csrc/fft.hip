@@ -589,8 +589,11 @@ static constexpr size_t LDS_BUDGET = 150 * 1024;
 // Radix order: 8s, a 4 or 2, then 7s, 5s, 3s.  With `edge`, a radix that
 // occurs twice (preferring 8, 5, 4, 7, 3) is moved to both ends, so the
 // first and last stages can run in registers; returns that radix in *re
-// (0 when none repeats).
-static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullptr) {
+// (0 when none repeats).  With `odd_first` (and no edge), the largest odd
+// radix goes first: the first stage writes each butterfly's outputs R apart,
+// and for the interleaved row lines an even R (8: 256-byte strides) sends a
+// wave's writes to one LDS bank window, while an odd R spreads them.
+static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullptr, bool odd_first = false) {
     int n = L, k = 0;
     int rad[UPDC_MAX_STAGES + 8];
     auto push = [&](int r) { if (k < UPDC_MAX_STAGES) rad[k++] = r; };
@@ -615,6 +618,17 @@ static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullpt
             for (int i = 0; i < m; ++i) rad[1 + i] = mid[i];
             rad[k - 1] = cand;
             if (re) *re = cand;
+            break;
+        }
+    }
+    if (odd_first && !(edge && re && *re)) {
+        for (int cand : {7, 5, 3}) {
+            int at = -1;
+            for (int i = 0; i < k; ++i)
+                if (rad[i] == cand) { at = i; break; }
+            if (at < 0) continue;
+            for (int i = at; i > 0; --i) rad[i] = rad[i - 1];
+            rad[0] = cand;
             break;
         }
     }
@@ -730,7 +744,7 @@ template <class T, class TA, int MODE> static void *col_e_fn(int re) {
 template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (FW < 2 || FH < 1 || (FW & 1)) return "FFT width must be even and >= 2";
     s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1;
-    if (!factor_radices(s.M, row)) return "FFT width/2 is not 7-smooth";
+    if (!factor_radices(s.M, row, false, nullptr, row_odd_first)) return "FFT width/2 is not 7-smooth";
     if (!factor_radices(FH, col, allow_edge, &col_re)) return "FFT height is not 7-smooth";
     // the edge kernel holds EDGE_V / RE butterflies of each thread in registers
     if (col_re && FH / col_re > (EDGE_V / col_re) * ColCfg<T>::NT) col_re = 0;

@@ -33,6 +33,7 @@ template <class T> struct Fft2 {
     int col_re = 0;          // edge radix of the column plan (0: LDS-only column kernel)
     bool allow_edge = true;  // set before init(): false forces the LDS-only column kernel
     bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit
+    bool row_odd_first = true;   // set before init(): an odd radix first in the row plan (LDS banks)
     int row_grid = 0;            // workgroups of the persistent row passes (one per CU)
 
     // Returns "" on success or an error message.
