@@ -478,9 +478,9 @@ def main() -> int:
         log(f"reference f32 vs FP64 after {K}: norm_linf={errr['norm_linf']:.3e} rel_rms={errr['rel_rms']:.3e}")
 
     # roofline of the dominant kernel
-    e, c, ed = 4, 8, 8
+    e, c, ed, ea = 4, 8, 8, 8
     sb = metrics.stage_bytes(ds.W, ds.H, FW, FH, [i["nin"] for i in infos], [i["nlinks"] for i in infos],
-                             [i["ncsc"] for i in infos], [i["linkmap_bytes"] for i in infos], e, c, ed)
+                             [i["ncsc"] for i in infos], [i["linkmap_bytes"] for i in infos], e, c, ed, ea)
     kern = {k: v for k, v in prof.items() if k in sb}
     dom = max(kern, key=lambda k: kern[k][0])
     ms = kern[dom][0]

@@ -394,7 +394,7 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
             fft_lds<true, T, NT, VMAX>(buf, f, 0);
             store(out + (size_t)k * out_bs);
         }
-    } else {  // COL_ADJ
+    } else {  // COL_ADJ, COL_ACC
         Cx<TA> acc[XMAX];
 #pragma unroll
         for (int j = 0; j < XMAX; ++j) acc[j] = cx<TA>((TA)0, (TA)0);
@@ -416,6 +416,16 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
                     acc[j].im += (TA)pim;
                 }
             }
+        }
+        if (MODE == COL_ACC) {
+            // the sum itself, in TA, for an inverse transform in TA
+            Cx<TA> *o = reinterpret_cast<Cx<TA> *>(out) + (size_t)col * out_rp;
+#pragma unroll
+            for (int j = 0; j < XMAX; ++j) {
+                const int e = tid + j * NT;
+                if (e < hout) o[e] = acc[j];
+            }
+            return;
         }
         __syncthreads();
 #pragma unroll
@@ -512,7 +522,7 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
             inv();
             store(out + (size_t)k * out_bs + (size_t)col * out_rp);
         }
-    } else {  // COL_ADJ
+    } else {  // COL_ADJ, COL_ACC
         Cx<TA> acc[NBE][RE];
 #pragma unroll
         for (int r = 0; r < NBE; ++r)
@@ -535,6 +545,18 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
                     acc[r][q].re += (TA)pre;
                     acc[r][q].im += (TA)pim;
                 }
+        }
+        if (MODE == COL_ACC) {
+            // the sum itself, in TA, for an inverse transform in TA
+            Cx<TA> *o = reinterpret_cast<Cx<TA> *>(out) + (size_t)col * out_rp;
+#pragma unroll
+            for (int r = 0; r < NBE; ++r)
+#pragma unroll
+                for (int q = 0; q < RE; ++q) {
+                    const int i = tid + r * NT, n = i + q * LR;
+                    if (i < LR && n < hout) o[n] = acc[r][q];
+                }
+            return;
         }
 #pragma unroll
         for (int r = 0; r < NBE; ++r)
@@ -737,8 +759,14 @@ template <class T, class TA, int MODE, bool PF> static void *col_e_fn_pf(int re)
     }
 }
 template <class T, class TA, int MODE> static void *col_e_fn(int re) {
-    constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ);
+    constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ || MODE == COL_ACC);
     return col_e_fn_pf<T, TA, MODE, PF>(re);
+}
+
+// COL_ACC kernels exist only for a sum wider than the transform type.
+template <class T, class TA> static void *col_acc_e_fn(int re) {
+    if constexpr (sizeof(TA) > sizeof(T)) return col_e_fn<T, TA, COL_ACC>(re);
+    return nullptr;
 }
 
 template <class T> std::string Fft2<T>::init(int FW, int FH) {
@@ -793,6 +821,11 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
         for (void *fn : fns_e)
             if (!fn || hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
                 return "cannot raise LDS limit for the column pass";
+    }
+    if constexpr (sizeof(T) < sizeof(double)) {
+        void *fn_acc = col_re ? col_acc_e_fn<T, double>(col_re) : col_fn<T, double, COL_ACC>();
+        if (!fn_acc || hipFuncSetAttribute(fn_acc, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
+            return "cannot raise LDS limit for the column pass";
     }
     return "";
 }
@@ -861,7 +894,9 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in
         void *fn = mode == COL_FWD     ? col_e_fn<T, T, COL_FWD>(col_re)
                    : mode == COL_INV   ? col_e_fn<T, T, COL_INV>(col_re)
                    : mode == COL_BCAST ? col_e_fn<T, T, COL_BCAST>(col_re)
-                                       : col_e_fn<T, TA, COL_ADJ>(col_re);
+                   : mode == COL_ADJ   ? col_e_fn<T, TA, COL_ADJ>(col_re)
+                                       : col_acc_e_fn<T, TA>(col_re);
+        if (!fn) return hipErrorInvalidValue;
         const int nc = s.NC, nexp = batch;
         const Cx<T> *pin = in, *ppsf = psf;
         Cx<T> *pout = out;
@@ -886,12 +921,30 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in
             hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_BCAST>), dim3(gx, 1), dim3(CNT), lds, st, col, s.NC,
                                in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch);
             break;
-        default:
+        case COL_ADJ:
             hipLaunchKernelGGL((k_col<T, TA, CNT, CVMAX, CXMAX, COL_ADJ>), dim3(gx, 1), dim3(CNT), lds, st, col, s.NC,
                                in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch);
             break;
+        default:
+            if constexpr (sizeof(TA) > sizeof(T)) {
+                hipLaunchKernelGGL((k_col<T, TA, CNT, CVMAX, CXMAX, COL_ACC>), dim3(gx, 1), dim3(CNT), lds, st, col,
+                                   s.NC, in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout,
+                                   batch);
+            } else {
+                return hipErrorInvalidValue;
+            }
+            break;
     }
     return hipGetLastError();
+}
+
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, const Cx<T> *psf,
+                           size_t psf_bstride, int hin, int batch, hipStream_t st) {
+    // the kernels write Cx<TA> elements at pitch rp(FH) through `out`
+    return colpass<TA>(COL_ACC, in, in_bstride, in_rows, reinterpret_cast<Cx<T> *>(out), 0, s.FH, psf, psf_bstride,
+                       hin, s.FH, batch, st);
 }
 
 template <class T> hipError_t Fft2<T>::spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st) {
@@ -931,5 +984,7 @@ template hipError_t Fft2<float>::colpass<double>(int, const Cx<float> *, size_t,
                                                  const Cx<float> *, size_t, int, int, int, hipStream_t);
 template hipError_t Fft2<double>::colpass<double>(int, const Cx<double> *, size_t, int, Cx<double> *, size_t, int,
                                                   const Cx<double> *, size_t, int, int, int, hipStream_t);
+template hipError_t Fft2<float>::colacc<double>(const Cx<float> *, size_t, int, Cx<double> *, const Cx<float> *,
+                                                size_t, int, int, hipStream_t);
 
 }  // namespace updc

@@ -16,7 +16,7 @@
 
 namespace updc {
 
-enum ColMode { COL_FWD = 0, COL_INV = 1, COL_BCAST = 2, COL_ADJ = 3 };
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_BCAST = 2, COL_ADJ = 3, COL_ACC = 4 };
 enum RowMode { ROW_PLAIN = 0, ROW_GATHER = 1 };
 
 struct FftShape {
@@ -62,6 +62,11 @@ template <class T> struct Fft2 {
     hipError_t colpass(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out, size_t out_bstride,
                        int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin, int hout, int batch,
                        hipStream_t st);
+    // COL_ACC: out = sum_k conj(psf[k]) * FFT(in[k]) kept in TA (TA wider than
+    // T), all FH rows, pitch rp(FH), for an inverse transform in TA.
+    template <class TA>
+    hipError_t colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, const Cx<T> *psf,
+                      size_t psf_bstride, int hin, int batch, hipStream_t st);
     // Plain (per-exposure) form helpers over full column-major spectra (pitch rp(FH)):
     //   out = a * psf_k ;  acc += conj(psf_k) * a  (acc in TA) ;  out = (T) acc
     hipError_t spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st);

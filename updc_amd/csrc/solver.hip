@@ -295,15 +295,21 @@ struct Ctx {
 
 // ------------------------------------------------------------------ pipeline
 template <class TF, class TA> struct Pipe : PipeBase {
+    // Mixed tier (TA wider than TF): the summed adjoint spectrum stays in TA
+    // and its inverse transform, columns then rows, runs in TA (fa), so C is
+    // never rounded to TF.  The per-exposure transforms stay in TF.
+    static constexpr bool kWide = sizeof(TA) > sizeof(TF);
     Fft2<TF> fft;
+    Fft2<TA> fa;  // kWide: the TA transform of the summed adjoint spectrum (and of D)
     DevBuf<ExpDev> d_ex;
     std::vector<ExpDev> h_ex;
-    DevBuf<TF> sci, w, wr, blur, x, C, cov;
+    DevBuf<TF> sci, w, wr, blur, x, cov;
     DevBuf<TF> x_init;  // the initial estimate, kept for updc_get_initial_f64
-    DevBuf<TA> D;
+    DevBuf<TA> C, D;
     DevBuf<uint8_t> frozen;
     DevBuf<Cx<TF>> psf_spec, spec_x, spec_k, spec_tmp;
-    DevBuf<Cx<TA>> spec_acc;  // per-exposure form: summed adjoint spectrum
+    DevBuf<Cx<TA>> spec_acc;  // summed adjoint spectrum (per-exposure form; kWide: every form), pitch rp(FH)
+    DevBuf<Cx<TA>> spec_xa;   // kWide: spec_acc after the TA column inverse, rows [0, H)
     DevBuf<ExpDiag> diag_cur, diag_hist;
     DevBuf<IterDiag> idiag_cur, idiag_hist;
     DevBuf<double> partials;
@@ -317,10 +323,33 @@ template <class TF, class TA> struct Pipe : PipeBase {
     ~Pipe() override {
         if (graph) (void)hipGraphExecDestroy(graph);
         fft.release();
+        fa.release();
     }
 
     int N(Ctx &c) const { return (int)c.ex.size(); }
     TF scale(Ctx &c) const { return (TF)1 / (TF)((size_t)c.FW * c.FH); }
+    TA scale_a(Ctx &c) const { return (TA)1 / (TA)((size_t)c.FW * c.FH); }
+
+    // C = crop C2R(sum_k conj(P^_k) spec_k) / nf from the row spectra spec_k
+    // of the n exposures (updc_core.c:985-995, 1280-1287).  Records one event
+    // after each launch but the last.
+    template <class Mark> void adjoint_sum(Ctx &c, int n, Mark mark) {
+        const FftShape &s = fft.s;
+        const size_t kstride = fft.spec_elems(c.H), fstride = fft.spec_elems(s.FH);
+        if constexpr (kWide) {
+            HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_acc.p, psf_spec.p, fstride, c.H, n, c.st));
+            mark();
+            HIPCHK(fa.template colpass<TA>(COL_INV, spec_acc.p, 0, s.FH, spec_xa.p, 0, c.H, nullptr, 0, s.FH, c.H, 1,
+                                           c.st));
+            mark();
+            HIPCHK(fa.row_c2r(spec_xa.p, 0, c.H, C.p, 0, c.W, c.W, scale_a(c), c.H, 1, c.st));
+        } else {
+            HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p, fstride,
+                                            c.H, c.H, n, c.st));
+            mark();
+            HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
+        }
+    }
 
     // ---- PSF spectra P^_k for every exposure into `out` (column-major,
     // N x f.spec_elems(FH), pitch rp(FH)).
@@ -418,6 +447,12 @@ template <class TF, class TA> struct Pipe : PipeBase {
         fft.allow_edge = c.col_edge != 0;
         std::string e = fft.init(c.FW, c.FH);
         if (!e.empty()) throw Error(UPDC_ERR_ARG, e);
+        if (kWide) {
+            // edge-stage columns where possible: the FP64 LDS-only kernel stops at FH 6144
+            fa.release();
+            e = fa.init(c.FW, c.FH);
+            if (!e.empty()) throw Error(UPDC_ERR_ARG, e);
+        }
         const FftShape &s = fft.s;
         max_nin = 0;
         for (auto &x : c.ex) max_nin = std::max<uint32_t>(max_nin, (uint32_t)x->nin);
@@ -474,6 +509,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const bool cache = !c.setup_cache_dir.empty() && c.prec == UPDC_PREC_F32;
         c.setup_cache_hits = 0;
         psf_spec.alloc((size_t)n * fft.spec_elems(s.FH));
+        DevBuf<Cx<TA>> psf_wide;  // kWide: the TA spectra, for D
         {
             const size_t P = rp(s.FH), one = fft.spec_elems(s.FH), nc = (size_t)s.FH * s.NC;
             std::vector<std::string> paths(n);
@@ -497,6 +533,12 @@ template <class TF, class TA> struct Pipe : PipeBase {
             }
             if (all) {
                 c.setup_cache_hits |= 1;
+            } else if (kWide) {
+                // the TA spectra rounded once to TF: the per-exposure products
+                // use the nearest TF values of the spectra D is built from
+                psf_wide.alloc((size_t)n * fa.spec_elems(s.FH));
+                psf_spectra<TA>(c, fa, psf_wide.p);
+                HIPCHK((convert<TA, TF>((const TA *)psf_wide.p, (TF *)psf_spec.p, 2 * psf_spec.n, c.st)));
             } else {
                 psf_spectra<TF>(c, fft, psf_spec.p);
                 if (cache) {
@@ -561,22 +603,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
             }
         } else {
             // mixed tier: the denominator is computed entirely in FP64
-            Fft2<double> f64;  // edge-stage columns where possible: the FP64 LDS-only kernel stops at FH 6144
-            std::string e2 = f64.init(c.FW, c.FH);
-            if (!e2.empty()) throw Error(UPDC_ERR_ARG, e2);
-            try {
-                DevBuf<Cx<double>> ps;
-                ps.alloc((size_t)n * f64.spec_elems(f64.s.FH));
-                psf_spectra<double>(c, f64, ps.p);
-                DevBuf<double> wd;
-                wd.alloc(npix);
-                HIPCHK((convert<TF, double>(w.p, wd.p, npix, c.st)));
-                denominator<double>(c, f64, ps.p, d_ex.p, wd.p, (double *)D.p);
-            } catch (...) {
-                f64.release();
-                throw;
-            }
-            f64.release();
+            DevBuf<TA> wd;
+            wd.alloc(npix);
+            HIPCHK((convert<TF, TA>(w.p, wd.p, npix, c.st)));
+            denominator<TA>(c, fa, psf_wide.p, d_ex.p, wd.p, D.p);
+            psf_wide.reset();
         }
         HIPCHK(fill<unsigned long long>(key.p, 0ull, 1, c.st));
         HIPCHK(max_key<TA>(D.p, nout, key.p, c.st));
@@ -591,6 +622,10 @@ template <class TF, class TA> struct Pipe : PipeBase {
         C.alloc(nout);
         spec_x.alloc(fft.spec_elems(std::max(c.H, s.FH)));
         spec_k.alloc((size_t)n * fft.spec_elems(c.H));
+        if (kWide) {
+            spec_acc.alloc(fft.spec_elems(s.FH));
+            spec_xa.alloc(fa.spec_elems(c.H));
+        }
         HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, npix, c.st));
         diag_cur.alloc(n);
         idiag_cur.alloc(1);
@@ -619,7 +654,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         auto mark = [&](int i) { if (ev) HIPCHK(hipEventRecord(ev[i], c.st)); };
         if (!spec_tmp.p) {
             spec_tmp.alloc(fstride);
-            spec_acc.alloc(fstride);
+            if (!spec_acc.p) spec_acc.alloc(fstride);
         }
         Cx<TF> *xh = spec_x.p;  // full FH-row spectrum (pitch rp(FH)) after the column pass
         mark(0);
@@ -642,10 +677,16 @@ template <class TF, class TA> struct Pipe : PipeBase {
             HIPCHK(fft.template spec_acc<TA>(spec_tmp.p, pk, spec_acc.p, c.st));
         }
         mark(2);
-        HIPCHK(fft.template spec_cast<TA>(spec_acc.p, spec_tmp.p, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, s.FH, spec_x.p, 0, c.H, nullptr, 0, s.FH, c.H, 1,
-                                        c.st));
-        HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
+        if constexpr (kWide) {
+            HIPCHK(fa.template colpass<TA>(COL_INV, spec_acc.p, 0, s.FH, spec_xa.p, 0, c.H, nullptr, 0, s.FH, c.H, 1,
+                                           c.st));
+            HIPCHK(fa.row_c2r(spec_xa.p, 0, c.H, C.p, 0, c.W, c.W, scale_a(c), c.H, 1, c.st));
+        } else {
+            HIPCHK(fft.template spec_cast<TA>(spec_acc.p, spec_tmp.p, c.st));
+            HIPCHK(fft.template colpass<TA>(COL_INV, spec_tmp.p, 0, s.FH, spec_x.p, 0, c.H, nullptr, 0, s.FH, c.H, 1,
+                                            c.st));
+            HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
+        }
         mark(3);
         HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
         mark(4);
@@ -684,10 +725,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
         }
         mark(m++);
-        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p,
-                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
-        mark(m++);
-        HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, sc, c.H, 1, c.st));
+        adjoint_sum(c, n, [&]() { mark(m++); });
         mark(m++);
         HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
         mark(m++);
@@ -711,7 +749,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(fill<unsigned int>(ctr.p, 0u, 1, c.st));
         const bool use_graph = (c.flags & UPDC_FLAG_GRAPH) != 0;
         std::vector<hipEvent_t> ev;
-        const int nev = 10;
+        const int nev = 12;
         hipEvent_t it0, it1;
         HIPCHK(hipEventCreate(&it0));
         HIPCHK(hipEventCreate(&it1));
@@ -754,12 +792,14 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const bool split = (c.flags & UPDC_FLAG_SPLIT_ADJOINT) != 0;
         if (plain)
             stage_names = {"plain_xhat", "plain_exposures", "plain_adjoint_inverse", "update"};
-        else if (split)
-            stage_names = {"row_r2c_x",     "col_fwd_bcast", "row_c2r_blur",  "gather_freeze", "adjoint_image",
-                           "row_r2c_adj",   "col_adj_accum", "row_c2r_corr",  "update"};
-        else
-            stage_names = {"row_r2c_x",      "col_fwd_bcast", "row_c2r_blur", "gather_freeze",
-                           "row_r2c_gather", "col_adj_accum", "row_c2r_corr", "update"};
+        else {
+            stage_names = {"row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze"};
+            if (split) stage_names.insert(stage_names.end(), {"adjoint_image", "row_r2c_adj"});
+            else stage_names.push_back("row_r2c_gather");
+            stage_names.push_back("col_adj_accum");
+            if (kWide) stage_names.push_back("col_inv_corr");
+            stage_names.insert(stage_names.end(), {"row_c2r_corr", "update"});
+        }
         const int nst = (int)stage_names.size();
         stage_ms.assign(nst + 1, 0.0);
         stage_n.assign(nst + 1, 0);
@@ -943,9 +983,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const size_t kstride = fft.spec_elems(c.H);
         from_host_f64(v, c.npix(), wr.p, c);
         HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
-        HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p,
-                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
-        HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
+        adjoint_sum(c, n, []() {});
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(C.p, c.nout(), out, c);
     }

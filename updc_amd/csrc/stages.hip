@@ -774,7 +774,7 @@ static constexpr int UPD_NT = 256, UPD_BLOCKS = 1024;
 int update_blocks() { return UPD_BLOCKS; }
 
 template <class TF, class TA>
-__global__ __launch_bounds__(UPD_NT) void k_update(TF *__restrict__ x, const TF *__restrict__ C,
+__global__ __launch_bounds__(UPD_NT) void k_update(TF *__restrict__ x, const TA *__restrict__ C,
                                                    const TA *__restrict__ D, size_t nout, TA max_sens,
                                                    IterDiag *idiag, double *partials, unsigned long long *negs,
                                                    unsigned int *ticket) {
@@ -791,7 +791,7 @@ __global__ __launch_bounds__(UPD_NT) void k_update(TF *__restrict__ x, const TF 
         const TA d = D[t];
         TF xv = x[t];
         if (d > thr) {
-            const TA c = (TA)C[t];
+            const TA c = C[t];
             TA f = c / d;
             if (!isfinite(f)) { bad |= UPDC_E_FACTOR; badidx = badidx < t ? badidx : t; }
             if (f < (TA)0) {
@@ -854,7 +854,7 @@ __global__ __launch_bounds__(UPD_NT) void k_update(TF *__restrict__ x, const TF 
     }
 }
 template <class TF, class TA>
-hipError_t update(TF *x, const TF *C, const TA *D, size_t nout, TA max_sens, IterDiag *idiag, double *partials,
+hipError_t update(TF *x, const TA *C, const TA *D, size_t nout, TA max_sens, IterDiag *idiag, double *partials,
                   unsigned long long *negs, unsigned int *ticket, hipStream_t st) {
     hipLaunchKernelGGL((k_update<TF, TA>), dim3(UPD_BLOCKS), dim3(UPD_NT), 0, st, x, C, D, nout, max_sens, idiag,
                        partials, negs, ticket);
@@ -1060,7 +1060,7 @@ template hipError_t gather_freeze<double, double>(const ExpDev *, int, int, uint
                                                   IterDiag *, hipStream_t);
 template hipError_t update<float, float>(float *, const float *, const float *, size_t, float, IterDiag *, double *,
                                          unsigned long long *, unsigned int *, hipStream_t);
-template hipError_t update<float, double>(float *, const float *, const double *, size_t, double, IterDiag *,
+template hipError_t update<float, double>(float *, const double *, const double *, size_t, double, IterDiag *,
                                           double *, unsigned long long *, unsigned int *, hipStream_t);
 template hipError_t update<double, double>(double *, const double *, const double *, size_t, double, IterDiag *,
                                            double *, unsigned long long *, unsigned int *, hipStream_t);

@@ -68,8 +68,9 @@ hipError_t predict(const ExpDev *ex, int nexp, uint32_t max_nin, const TF *blur,
                    hipStream_t st);
 
 // ---- multiplicative update (updc_core.c:1437-1457) --------------------------
+// C and D in the accumulation type TA (the mixed tier keeps both in FP64).
 template <class TF, class TA>
-hipError_t update(TF *x, const TF *C, const TA *D, size_t nout, TA max_sens, IterDiag *idiag, double *partials,
+hipError_t update(TF *x, const TA *C, const TA *D, size_t nout, TA max_sens, IterDiag *idiag, double *partials,
                   unsigned long long *negs, unsigned int *ticket, hipStream_t st);
 int update_blocks();
 

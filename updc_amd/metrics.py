@@ -25,11 +25,14 @@ def stats(ref: np.ndarray, new: np.ndarray) -> dict:
 
 
 def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[int], ncsc: list[int],
-                lm_bytes: list[int], e: int, c: int, e_d: int) -> dict:
+                lm_bytes: list[int], e: int, c: int, e_d: int, e_a: int | None = None) -> dict:
     """Algorithmic HBM bytes per launch of each iteration kernel: every
     operand read once and every result written once (FFT-internal LDS passes
     and re-reads excluded).  e/c = real/complex element bytes of the FFT
-    type, e_d = bytes of the denominator element."""
+    type, e_d = bytes of the denominator element.  e_a: real element bytes of
+    a summed adjoint spectrum wider than the FFT type (the mixed tier), which
+    leaves the column pass whole (FH rows) and is inverted in that type by
+    col_inv_corr and row_c2r_corr."""
     N = len(nin)
     NC = FW // 2 + 1
     nout = W * H
@@ -39,7 +42,7 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
     pix = sum(nin)
     lm = sum(lm_bytes)
     csc = sum(4 * (nout + 1) + 8 * n for n in ncsc)
-    return {
+    out = {
         "row_r2c_x": nout * e + spec_x,
         "col_fwd_bcast": spec_x + psf + spec_k,
         "row_c2r_blur": spec_k + N * nout * e,
@@ -54,3 +57,14 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
         "row_c2r_corr": spec_x + nout * e,
         "update": nout * (3 * e + e_d),
     }
+    if e_a is not None and e_a > e:
+        acc_full = FH * NC * 2 * e_a
+        acc_h = H * NC * 2 * e_a
+        out.update({
+            "col_adj_accum": spec_k + psf + acc_full,
+            "col_inv_corr": acc_full + acc_h,
+            "row_c2r_corr": acc_h + nout * e_a,
+            # x read and written, C in the wide type, D
+            "update": nout * (2 * e + e_a + e_d),
+        })
+    return out

@@ -31,7 +31,7 @@ from pathlib import Path
 
 # iteration stage names, in launch order (solver.hip launch_iteration)
 STAGES = ("row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze", "row_r2c_gather", "adjoint_image",
-          "row_r2c_adj", "col_adj_accum", "row_c2r_corr", "update")
+          "row_r2c_adj", "col_adj_accum", "col_inv_corr", "row_c2r_corr", "update")
 
 
 def _template_args(name: str) -> list[str]:
@@ -39,20 +39,27 @@ def _template_args(name: str) -> list[str]:
     return [a.strip() for a in m.group(1).split(",")] if m else []
 
 
-def classify(name: str, grid_y: int) -> str | None:
-    """Iteration stage of a dispatch, from the kernel name and its grid's y
-    extent (the exposure batch), or None for other kernels."""
-    if "k_row_r2c<" in name:
-        mode = _template_args(name)[-1]
-        if mode == "1":
+def _row_r2c_gather(name: str) -> bool:
+    # k_row_r2c<T, NT, VMAX, RB, MODE>: MODE 1 is the fused CSC gather; the
+    # persistent k_row_r2c_p is the plain mode only
+    return "k_row_r2c<" in name and _template_args(name)[-1] == "1"
+
+
+def classify(name: str, prev: str | None) -> str | None:
+    """Iteration stage of a dispatch, from the kernel name and the stage of
+    the iteration dispatch before it, or None for other kernels."""
+    if "k_row_r2c" in name:
+        if _row_r2c_gather(name):
             return "row_r2c_gather"
-        return "row_r2c_x" if grid_y == 1 else "row_r2c_adj"
-    if "k_row_c2r<" in name:
-        return "row_c2r_corr" if grid_y == 1 else "row_c2r_blur"
+        return "row_r2c_adj" if prev == "adjoint_image" else "row_r2c_x"
+    if "k_row_c2r" in name:
+        return "row_c2r_blur" if prev == "col_fwd_bcast" else "row_c2r_corr"
     if "k_col_e<" in name or "k_col<" in name:
         args = _template_args(name)
         mode = args[5] if "k_col_e<" in name else args[-1]
-        return {"2": "col_fwd_bcast", "3": "col_adj_accum"}.get(mode)
+        # COL_INV (1) appears in the batched iteration only as the inverse of
+        # the summed adjoint spectrum kept in FP64
+        return {"1": "col_inv_corr", "2": "col_fwd_bcast", "3": "col_adj_accum", "4": "col_adj_accum"}.get(mode)
     if "k_gather<" in name:
         return "gather_freeze"
     if "k_csc_image<" in name:
@@ -69,19 +76,27 @@ def _find(d: Path, suffix: str) -> Path:
     return hits[0]
 
 
-def _iteration_window(rows: list[dict], key_name: str) -> list[dict]:
-    """Dispatches from the first iteration on: the row_r2c_x dispatch that
-    precedes the first gather opens the window."""
+def _iteration_stages(rows: list[dict], key_name: str) -> list[tuple[str, dict]]:
+    """(stage, row) of the dispatches from the first iteration on, in
+    dispatch order: the plain row R2C that precedes the first gather opens
+    the window."""
     rows = sorted(rows, key=lambda r: int(r["Dispatch_Id"]))
     first = next((i for i, r in enumerate(rows) if "k_gather<" in r[key_name]), None)
     if first is None:
         return []
     start = first
     for i in range(first, -1, -1):
-        if "k_row_r2c<" in rows[i][key_name] and _template_args(rows[i][key_name])[-1] == "0":
+        nm = rows[i][key_name]
+        if "k_row_r2c" in nm and not _row_r2c_gather(nm):
             start = i
             break
-    return rows[start:]
+    out, prev = [], None
+    for r in rows[start:]:
+        st = classify(r[key_name], prev)
+        if st is not None:
+            out.append((st, r))
+            prev = st
+    return out
 
 
 def trace_summary(d: Path, cmds: list[str]) -> str:
@@ -89,10 +104,7 @@ def trace_summary(d: Path, cmds: list[str]) -> str:
     rows = list(csv.DictReader(open(path)))
     per = defaultdict(list)
     names = {}
-    for r in _iteration_window(rows, "Kernel_Name"):
-        st = classify(r["Kernel_Name"], int(r["Grid_Size_Y"]) // max(1, int(r["Workgroup_Size_Y"])))
-        if st is None:
-            continue
+    for st, r in _iteration_stages(rows, "Kernel_Name"):
         per[st].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
         names[st] = r["Kernel_Name"].split("(")[0]
     total = sum(sum(v) for v in per.values())
@@ -118,17 +130,9 @@ def _counter_means(d: Path, counter: str) -> tuple[dict, float]:
     copy = [float(r["Counter_Value"]) for r in rows if "k_copy_ref" in r["Kernel_Name"]]
     if not copy:
         raise SystemExit(f"{path}: no k_copy_ref dispatch to calibrate {counter}")
-    # the batch of a dispatch: grid size relative to the smallest grid of the same kernel
-    it = _iteration_window(rows, "Kernel_Name")
-    gmin = defaultdict(lambda: 1 << 62)
-    for r in it:
-        gmin[r["Kernel_Name"]] = min(gmin[r["Kernel_Name"]], int(r["Grid_Size"]))
     per = defaultdict(list)
-    for r in it:
-        y = int(r["Grid_Size"]) // gmin[r["Kernel_Name"]]
-        st = classify(r["Kernel_Name"], y)
-        if st is not None:
-            per[st].append(float(r["Counter_Value"]))
+    for st, r in _iteration_stages(rows, "Kernel_Name"):
+        per[st].append(float(r["Counter_Value"]))
     return {k: statistics.fmean(v) for k, v in per.items()}, statistics.fmean(copy)
 
 
