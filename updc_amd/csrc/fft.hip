@@ -451,13 +451,14 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
 // butterflies, twiddles and their order are those of the LDS-only kernel.
 // PF: the next PSF column (BCAST) or the current one (ADJ) is loaded into
 // registers before the transform that precedes its use.
-static constexpr int EDGE_V = 16;  // complex values per thread per register set
-template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF>
+// EV: complex values per thread per register set (the edge butterflies of
+// one thread, EV / RE of them).
+template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF, int EV>
 __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__restrict__ in, size_t in_bs, int in_rp,
                                               Cx<T> *__restrict__ out, size_t out_bs, int out_rp,
                                               const Cx<T> *__restrict__ psf, size_t psf_bs, int psf_rp, int hin,
                                               int hout, int nexp) {
-    constexpr int NBE = EDGE_V / RE;
+    constexpr int NBE = EV / RE;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
     const int col = xcd_tile(blockIdx.x, nc);
@@ -604,8 +605,15 @@ template <> struct RowCfg<float> { static constexpr int NT = 1024, VMAX = 16, RB
 template <> struct RowCfg<double> { static constexpr int NT = 512, VMAX = 16, RB = 2, PFM = 16, PFR = 8; };
 template <class T> struct ColCfg;
 // PF: PSF prefetch in the edge column kernel (FP64 would spill with it).
-template <> struct ColCfg<float> { static constexpr int NT = 512, VMAX = 16, XMAX = 16; static constexpr bool PF = true; };
-template <> struct ColCfg<double> { static constexpr int NT = 512, VMAX = 16, XMAX = 12; static constexpr bool PF = false; };
+// EV: complex values per thread of the edge kernel's register sets.
+template <> struct ColCfg<float> {
+    static constexpr int NT = 1024, VMAX = 10, XMAX = 8, EV = 10;
+    static constexpr bool PF = true;
+};
+template <> struct ColCfg<double> {
+    static constexpr int NT = 512, VMAX = 16, XMAX = 12, EV = 16;
+    static constexpr bool PF = false;
+};
 static constexpr size_t LDS_BUDGET = 150 * 1024;
 
 // Radix order: 8s, a 4 or 2, then 7s, 5s, 3s.  With `edge`, a radix that
@@ -748,13 +756,13 @@ template <class T, class TA, int MODE> static void *col_fn() {
 }
 
 template <class T, class TA, int MODE, bool PF> static void *col_e_fn_pf(int re) {
-    constexpr int NT = ColCfg<T>::NT, V = ColCfg<T>::VMAX;
+    constexpr int NT = ColCfg<T>::NT, V = ColCfg<T>::VMAX, EV = ColCfg<T>::EV;
     switch (re) {
-        case 3: return (void *)k_col_e<T, TA, NT, V, 3, MODE, PF>;
-        case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF>;
-        case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF>;
-        case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF>;
-        case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF>;
+        case 3: return (void *)k_col_e<T, TA, NT, V, 3, MODE, PF, EV>;
+        case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF, EV>;
+        case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF, EV>;
+        case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF, EV>;
+        case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF, EV>;
         default: return nullptr;
     }
 }
@@ -774,8 +782,8 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1;
     if (!factor_radices(s.M, row, false, nullptr, row_odd_first)) return "FFT width/2 is not 7-smooth";
     if (!factor_radices(FH, col, allow_edge, &col_re)) return "FFT height is not 7-smooth";
-    // the edge kernel holds EDGE_V / RE butterflies of each thread in registers
-    if (col_re && FH / col_re > (EDGE_V / col_re) * ColCfg<T>::NT) col_re = 0;
+    // the edge kernel holds EV / RE butterflies of each thread in registers
+    if (col_re && FH / col_re > (ColCfg<T>::EV / col_re) * ColCfg<T>::NT) col_re = 0;
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
     if (row_lds<T>(s.M) > LDS_BUDGET || !fits(row, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX))
         return "FFT width too large for the LDS-resident row pass";
