@@ -18,8 +18,11 @@ namespace updc {
 
 // ------------------------------------------------------------------ kernels
 // Row gather batching: RG_GB targets per lane per batch, the first RG_GU CSC
-// entries of each loaded together (further entries, rare, one at a time).
-static constexpr int RG_GB = 8, RG_GU = 3;
+// entries of each loaded together, further entries one at a time.  A target
+// pixel overlaps at most 4 pixels of a detector twice as coarse, so at that
+// scale no entry takes the one-at-a-time path; 6 x 4 is the widest batch that
+// fits the 128 VGPRs of a 1024-thread workgroup without spills.
+static constexpr int RG_GB = 6, RG_GU = 4;
 
 // A real row's (x[2n], x[2n+1]) pair as one 8-byte (FP32) or 16-byte (FP64)
 // access when `vec` (the host checked an even pitch and batch stride and an
