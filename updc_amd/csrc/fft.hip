@@ -23,6 +23,7 @@ namespace updc {
 // scale no entry takes the one-at-a-time path; 6 x 4 is the widest batch that
 // fits the 128 VGPRs of a 1024-thread workgroup without spills.
 static constexpr int RG_GB = 6, RG_GU = 4;
+template <class T> constexpr int row_gather_gb() { return sizeof(T) == 4 ? RG_GB : 4; }  // FP64: the wv values take twice the VGPRs
 
 // A real row's (x[2n], x[2n+1]) pair as one 8-byte (FP32) or 16-byte (FP64)
 // access when `vec` (the host checked an even pitch and batch stride and an
@@ -54,9 +55,9 @@ template <class T> __device__ __forceinline__ void store_pair(T *row, int c, int
     }
 }
 
-// Minimum waves per SIMD of the row kernels: FP32 keeps two 512-thread
-// workgroups per CU (<= 128 VGPRs), so one workgroup's loads overlap the
-// other's transform; FP64 keeps one.
+// Minimum waves per SIMD requested of the row kernels.  Their 1024-thread
+// workgroups (one per CU: 89.6 KB of LDS at M = 2800) already hold them to
+// four waves per SIMD and 128 VGPRs; the request adds no further limit.
 template <class T, int NT> constexpr int row_min_waves() { return sizeof(T) == 4 ? 2048 / NT : 1024 / NT; }
 
 // Row R2C: RB consecutive rows per workgroup (interleaved in LDS, row j at
@@ -91,6 +92,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen
         // complex slot ((c/2) << LG) + j.  Lanes take consecutive targets; a
         // batch of GB targets per lane issues its offset loads, then its
         // entry loads, then its w loads together (three round trips per batch).
+        constexpr int GB = row_gather_gb<T>();
         const ExpDev &e = ex[b];
         const T *w = wr + e.pix;
         T *zf = reinterpret_cast<T *>(z);
@@ -98,11 +100,11 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen
         for (int j = 0; j < RB; ++j)
             for (int c = wc + threadIdx.x; c < 2 * M; c += NT) zf[2 * (((c >> 1) << LG) + j) + (c & 1)] = (T)0;
         const int per_row = (wc + NT - 1) / NT, total = RB * per_row;
-        for (int u0 = 0; u0 < total; u0 += RG_GB) {
-            uint32_t a[RG_GB], cnt[RG_GB];
-            int slot[RG_GB];
+        for (int u0 = 0; u0 < total; u0 += GB) {
+            uint32_t a[GB], cnt[GB];
+            int slot[GB];
 #pragma unroll
-            for (int g = 0; g < RG_GB; ++g) {
+            for (int g = 0; g < GB; ++g) {
                 const int u = u0 + g, j = u / per_row, c = threadIdx.x + (u - j * per_row) * NT, r = r0 + j;
                 slot[g] = (u < total && c < wc) ? 2 * (((c >> 1) << LG) + j) + (c & 1) : -1;
                 const bool ok = slot[g] >= 0 && r < rows;
@@ -110,20 +112,20 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen
                 a[g] = ok ? e.toff[t] : 0u;
                 cnt[g] = ok ? e.toff[t + 1] : 0u;
             }
-            uint2 en[RG_GB][RG_GU];
-            T wv[RG_GB][RG_GU];
+            uint2 en[GB][RG_GU];
+            T wv[GB][RG_GU];
 #pragma unroll
-            for (int g = 0; g < RG_GB; ++g) {
+            for (int g = 0; g < GB; ++g) {
                 cnt[g] -= a[g];
 #pragma unroll
                 for (int k = 0; k < RG_GU; ++k) en[g][k] = (uint32_t)k < cnt[g] ? e.tent[a[g] + k] : make_uint2(0u, 0u);
             }
 #pragma unroll
-            for (int g = 0; g < RG_GB; ++g)
+            for (int g = 0; g < GB; ++g)
 #pragma unroll
                 for (int k = 0; k < RG_GU; ++k) wv[g][k] = (uint32_t)k < cnt[g] ? w[en[g][k].x] : (T)0;
 #pragma unroll
-            for (int g = 0; g < RG_GB; ++g) {
+            for (int g = 0; g < GB; ++g) {
                 // scatter[t] = sum over source pixels p (ascending) of fma(frac, w[p], .)
                 // from 0 -- the fmaf chain of updc_core.c:1173-1176 / :1064, since
                 // the reference visits pixels in ascending p and each (p,t) pair once
@@ -605,7 +607,7 @@ template <class T> struct RowCfg;
 // PFM / PFR: prefetch registers of the persistent row kernels; shapes beyond
 // them (FP32: M > 3071) use the one-tile-per-workgroup kernels.
 template <> struct RowCfg<float> { static constexpr int NT = 1024, VMAX = 16, RB = 4, PFM = 12, PFR = 4; };
-template <> struct RowCfg<double> { static constexpr int NT = 512, VMAX = 16, RB = 2, PFM = 16, PFR = 8; };
+template <> struct RowCfg<double> { static constexpr int NT = 1024, VMAX = 10, RB = 2, PFM = 6, PFR = 3; };
 template <class T> struct ColCfg;
 // PF: PSF prefetch in the edge column kernel (FP64 would spill with it).
 // EV: complex values per thread of the edge kernel's register sets.
