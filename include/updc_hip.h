@@ -135,9 +135,11 @@ int updc_ctx_set_preprocess_workers(updc_ctx *ctx, int workers);
  * updc_core.c:170-291).  Candidates: the first five admissible sizes per axis
  * from the minimum (even 7-smooth widths, 7-smooth heights), the diagonal
  * pairs, then (w2,h0), (w0,h2), (w4,h0), (w0,h4); up to 9 shapes.  Each is
- * timed as the batched round trip of the iteration's transforms (row R2C,
- * column forward, column inverse, row C2R over all exposures), with the
- * edge-stage and the LDS-only column kernel. */
+ * timed, with the edge-stage and the LDS-only column kernel, on the
+ * transforms of one batched iteration: X^, the broadcast column pass with the
+ * PSF products over all exposures, their row inverses, the exposures' row
+ * R2Cs, the summed adjoint column pass and the final row C2R (the reference
+ * times one R2C + C2R round trip instead). */
 int updc_fft_shape_candidates(int min_width, int min_height, int *widths, int *heights, int capacity,
                               int *count);
 /* Cache directory of the autotune result (the reference keeps

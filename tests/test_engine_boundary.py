@@ -171,7 +171,7 @@ def test_dropin_persistent_caches(small_ds, tmp_path):
     manifest, _ = _files(ds, tmp_path)
     env = E.solver_env(True)
     a = E.run_core(_argv(ds, tmp_path, manifest, 2, 0, prefix="a", autotune=True), env=env, timeout=120)
-    assert list(tmp_path.glob("fft_autotune_gpu_v1_*.txt"))
+    assert list(tmp_path.glob("fft_autotune_gpu_v2_*.txt"))
     out = subprocess.run(_argv(ds, tmp_path, manifest, 2, 0, prefix="b", autotune=True), env=env,
                          capture_output=True, text=True, timeout=120, check=True).stdout
     assert "persistent cache hit" in out and "FFT autotune persistent-cache hit" in out

@@ -93,7 +93,7 @@ def test_autotune_picks_a_candidate_and_caches(small_ds, tmp_path):
     timed = [r for r in rows if r[3] > 0]
     best = min(timed, key=lambda r: r[3])
     assert (best[0], best[1]) == shape
-    assert list(tmp_path.glob("fft_autotune_gpu_v1_f32_*.txt"))
+    assert list(tmp_path.glob("fft_autotune_gpu_v2_f32_*.txt"))
     a.iterate(3)
     xa = a.estimate(f64=True)
     a.close()

@@ -264,7 +264,7 @@ static void save_atomic(const std::string &path, const void *src, size_t bytes) 
 
 struct TuneRow {
     int w, h, edge;  // shape and column kernel (1 edge-stage, 0 LDS-only)
-    double ms;       // batched round trip; < 0 when the shape is unsupported
+    double ms;       // the iteration's transforms; < 0 when the shape is unsupported
 };
 
 struct Ctx {
@@ -1241,11 +1241,17 @@ static int shape_candidates(int min_w, int min_h, int (*out)[2], int cap) {
     return n;
 }
 
-// Batched round trip of the iteration's transforms for one shape: row R2C
-// of N target-sized images, forward column pass, inverse column pass, row
-// C2R (the reference times one R2C + C2R, updc_core.c:225-240).  Best of
-// three after a warm-up, in ms; < 0 when the shape does not fit the kernels.
-template <class T> static double time_round_trip(Ctx &c, int FW, int FH, bool edge, bool *edge_used) {
+// The transforms of one batched iteration at shape FW x FH, timed (best of
+// three after a warm-up, in ms; < 0 when the shape does not fit the kernels):
+// X^ (row R2C of one image, then the broadcast column pass with the PSF
+// product and inverse for all n exposures), the n blurred rows back, the n
+// adjoint images forward (plain row R2C standing in for the gather-fed one),
+// the summed adjoint column pass and the final row C2R.  Values are zero; the
+// timing depends only on the shape.  This is the work the shape choice
+// changes, where the reference times one R2C + C2R round trip
+// (updc_core.c:225-240); the two rank shapes differently once the column
+// passes carry the exposure loop and the PSF products.
+template <class T> static double time_iteration_transforms(Ctx &c, int FW, int FH, bool edge, bool *edge_used) {
     Fft2<T> f;
     f.allow_edge = edge;
     if (!f.init(FW, FH).empty()) {
@@ -1259,18 +1265,23 @@ template <class T> static double time_round_trip(Ctx &c, int FW, int FH, bool ed
     hipEvent_t a = nullptr, b = nullptr;
     try {
         DevBuf<T> img;
-        DevBuf<Cx<T>> spec;
+        DevBuf<Cx<T>> sk, sx, ps;
+        const size_t kh = f.spec_elems(c.H), kf = f.spec_elems(FH);
         img.alloc((size_t)n * nimg);
-        spec.alloc((size_t)n * f.spec_elems(FH));
+        sk.alloc((size_t)n * kh);
+        sx.alloc(f.spec_elems(std::max(c.H, FH)));
+        ps.alloc((size_t)n * kf);
         HIPCHK(hipMemsetAsync(img.p, 0, (size_t)n * nimg * sizeof(T), c.st));
+        HIPCHK(hipMemsetAsync(ps.p, 0, (size_t)n * kf * sizeof(Cx<T>), c.st));
         HIPCHK(hipEventCreate(&a));
         HIPCHK(hipEventCreate(&b));
-        const size_t ss = f.spec_elems(FH);
         auto trip = [&]() {
-            HIPCHK(f.row_r2c(ROW_PLAIN, img.p, nimg, c.W, c.W, nullptr, nullptr, 0, spec.p, ss, FH, c.H, n, c.st));
-            HIPCHK(f.template colpass<T>(COL_FWD, spec.p, ss, FH, spec.p, ss, FH, nullptr, 0, c.H, FH, n, c.st));
-            HIPCHK(f.template colpass<T>(COL_INV, spec.p, ss, FH, spec.p, ss, FH, nullptr, 0, FH, c.H, n, c.st));
-            HIPCHK(f.row_c2r(spec.p, ss, FH, img.p, nimg, c.W, c.W, (T)1, c.H, n, c.st));
+            HIPCHK(f.row_r2c(ROW_PLAIN, img.p, 0, c.W, c.W, nullptr, nullptr, 0, sx.p, 0, c.H, c.H, 1, c.st));
+            HIPCHK(f.template colpass<T>(COL_BCAST, sx.p, 0, c.H, sk.p, kh, c.H, ps.p, kf, c.H, c.H, n, c.st));
+            HIPCHK(f.row_c2r(sk.p, kh, c.H, img.p, nimg, c.W, c.W, (T)1, c.H, n, c.st));
+            HIPCHK(f.row_r2c(ROW_PLAIN, img.p, nimg, c.W, c.W, nullptr, nullptr, 0, sk.p, kh, c.H, c.H, n, c.st));
+            HIPCHK(f.template colpass<T>(COL_ADJ, sk.p, kh, c.H, sx.p, 0, c.H, ps.p, kf, c.H, c.H, n, c.st));
+            HIPCHK(f.row_c2r(sx.p, 0, c.H, img.p, 0, c.W, c.W, (T)1, c.H, 1, c.st));
         };
         trip();
         for (int r = 0; r < 3; ++r) {
@@ -1302,7 +1313,7 @@ static std::string device_signature(int dev) {
 
 static std::string tune_cache_path(const Ctx &c, int min_w, int min_h) {
     const char *pn = c.prec == UPDC_PREC_F64 ? "f64" : c.prec == UPDC_PREC_MIXED ? "mixed" : "f32";
-    return c.fft_cache_dir + "/fft_autotune_gpu_v1_" + pn + "_n" + std::to_string(c.ex.size()) + "_" +
+    return c.fft_cache_dir + "/fft_autotune_gpu_v2_" + pn + "_n" + std::to_string(c.ex.size()) + "_" +
            std::to_string(c.W) + "x" + std::to_string(c.H) + "_" + std::to_string(min_w) + "x" +
            std::to_string(min_h) + ".txt";
 }
@@ -1314,7 +1325,7 @@ static bool load_tune_cache(Ctx &c, int min_w, int min_h) {
     char sig[512] = {0};
     int w = 0, h = 0, edge = 0;
     double ms = 0;
-    const bool ok = std::fscanf(f, "signature=%511[^\n]\nwidth=%d\nheight=%d\ncolumn_edge=%d\nroundtrip_ms=%lf",
+    const bool ok = std::fscanf(f, "signature=%511[^\n]\nwidth=%d\nheight=%d\ncolumn_edge=%d\ntransforms_ms=%lf",
                                 sig, &w, &h, &edge, &ms) == 5;
     std::fclose(f);
     if (!ok || device_signature(c.device) != sig || w < min_w || h < min_h || (w & 1) || !seven_smooth(w) ||
@@ -1334,7 +1345,7 @@ static void save_tune_cache(const Ctx &c, int min_w, int min_h, double ms) {
     const std::string tmp = path + ".tmp." + std::to_string((long)getpid());
     FILE *f = std::fopen(tmp.c_str(), "w");
     if (!f) return;
-    std::fprintf(f, "signature=%s\nwidth=%d\nheight=%d\ncolumn_edge=%d\nroundtrip_ms=%.6f\n",
+    std::fprintf(f, "signature=%s\nwidth=%d\nheight=%d\ncolumn_edge=%d\ntransforms_ms=%.6f\n",
                  device_signature(c.device).c_str(), c.FW, c.FH, c.col_edge, ms);
     if (std::fclose(f) != 0 || std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
 }
@@ -1342,7 +1353,7 @@ static void save_tune_cache(const Ctx &c, int min_w, int min_h, double ms) {
 static void autotune_shape(Ctx &c, int min_w, int min_h) {
     if (load_tune_cache(c, min_w, min_h)) {
         if (!c.quiet())
-            std::printf("FFT autotune persistent-cache hit: %dx%d, %s column kernel (roundtrip %.3f ms)\n", c.FW,
+            std::printf("FFT autotune persistent-cache hit: %dx%d, %s column kernel (iteration transforms %.3f ms)\n", c.FW,
                         c.FH, c.col_edge ? "edge-stage" : "LDS-only", c.tune[0].ms);
         return;
     }
@@ -1357,19 +1368,19 @@ static void autotune_shape(Ctx &c, int min_w, int min_h) {
         for (int edge = 1; edge >= 0; --edge) {
             bool used = false;
             const bool f64 = c.prec == UPDC_PREC_F64;
-            const double ms = f64 ? time_round_trip<double>(c, shapes[i][0], shapes[i][1], edge != 0, &used)
-                                  : time_round_trip<float>(c, shapes[i][0], shapes[i][1], edge != 0, &used);
+            const double ms = f64 ? time_iteration_transforms<double>(c, shapes[i][0], shapes[i][1], edge != 0, &used)
+                                  : time_iteration_transforms<float>(c, shapes[i][0], shapes[i][1], edge != 0, &used);
             if (edge && !used && ms >= 0) {
                 // no edge radix for this height: the LDS-only kernel ran; record it once
                 c.tune.push_back(TuneRow{shapes[i][0], shapes[i][1], 0, ms});
                 if (best < 0 || ms < best) { best = ms; bw = shapes[i][0]; bh = shapes[i][1]; be = 0; }
-                if (!c.quiet()) std::printf("  FFT candidate %dx%d LDS-only roundtrip %.3f ms\n", shapes[i][0], shapes[i][1], ms);
+                if (!c.quiet()) std::printf("  FFT candidate %dx%d LDS-only transforms %.3f ms\n", shapes[i][0], shapes[i][1], ms);
                 break;
             }
             c.tune.push_back(TuneRow{shapes[i][0], shapes[i][1], edge, ms});
             if (!c.quiet()) {
                 if (ms < 0) std::printf("  FFT candidate %dx%d unsupported by the LDS-resident passes\n", shapes[i][0], shapes[i][1]);
-                else std::printf("  FFT candidate %dx%d %s roundtrip %.3f ms\n", shapes[i][0], shapes[i][1],
+                else std::printf("  FFT candidate %dx%d %s transforms %.3f ms\n", shapes[i][0], shapes[i][1],
                                  edge ? "edge-stage" : "LDS-only", ms);
             }
             if (ms < 0) break;
@@ -1381,7 +1392,7 @@ static void autotune_shape(Ctx &c, int min_w, int min_h) {
     c.FH = bh;
     c.col_edge = be;
     if (!c.quiet())
-        std::printf("FFT autotune selected %dx%d, %s column kernel (roundtrip %.3f ms)\n", bw, bh,
+        std::printf("FFT autotune selected %dx%d, %s column kernel (iteration transforms %.3f ms)\n", bw, bh,
                     be ? "edge-stage" : "LDS-only", best);
     save_tune_cache(c, min_w, min_h, best);
 }
