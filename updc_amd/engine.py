@@ -146,3 +146,27 @@ def run_core(argv: Sequence[str], env: dict | None = None, timeout: float | None
         with tp.open() as f:
             timing = {r["stage"]: float(r["wall_seconds"]) for r in csv.DictReader(f)}
     return CoreResult(its, read_plane(Path(wht_path), width, height), rows, timing)
+
+
+def write_iteration_products(result: CoreResult, out_dir: Path, stem: str, target_cards, metadata,
+                             input_metrics: Sequence[dict], weight_mode: str, psf_records: Sequence[dict],
+                             psf_names: Sequence[str]) -> list[Path]:
+    """The engine's "write iteration FITS" and "validate output FITS" steps
+    (_engine.py:1416-1446): one <stem>_iterNN_i2d.fits per saved iteration,
+    with that iteration's FreezeMu rows, then every file re-read and checked.
+    FITS handling is updc_amd.fitsout (no astropy)."""
+    from . import fitsout
+    out_dir = Path(out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    files = []
+    for it in sorted(result.iterations):
+        name = f"{stem}_iter{it:02d}_i2d.fits"
+        rows = [r for r in result.freeze_rows if r["iteration"] == it]
+        files.append(fitsout.write_iteration_product(out_dir / name, result.iterations[it], result.wht, target_cards,
+                                                     metadata, input_metrics, it, name, weight_mode, rows,
+                                                     psf_records, psf_names))
+    if len(files) != len(result.iterations):
+        raise ValueError("Missing requested iteration FITS files")
+    for f in files:
+        fitsout.validate_product(f, len(input_metrics))
+    return files
