@@ -128,6 +128,27 @@ def test_dropin_executable_end_to_end(small_ds, tmp_path):
     # LinkMaps were written next to the vertices, in the ULM2 format
     assert all(Path(str(r.vertices) + ".links").exists() for r in rows)
 
+    print("case: FITS products of the core's outputs", flush=True)
+    from updc_amd import fitsout as F
+    n = len(ds.geoms)
+    metrics = [dict(source=f"exp{k}.fits", background=float(ds.background[k]), background_sigma=0.0,
+                    poisson_pedestal=float(ds.pedestal[k]), auto_positivity_pedestal=0.0,
+                    nonpositive_valid_before_pedestal=0, sky_var_poisson=0.0) for k in range(n)]
+    psf = [dict(anchor_x=0.0, anchor_y=0.0, oversample=1, center_target_x=0.0, center_target_y=0.0, extent_x=0.0,
+                extent_y=0.0, input_width=p.shape[1], input_height=p.shape[0], mapped_width=p.shape[1],
+                mapped_height=p.shape[0], output_width=p.shape[1], output_height=p.shape[0],
+                input_sum=float(p.sum()), output_sum=float(p.sum()), crop_retained_flux=1.0,
+                crop_discarded_flux=0.0, flux_rel_error=0.0, overlaps=0, method="NONE", wcs_used="")
+           for p in ds.psfs]
+    files = E.write_iteration_products(res, tmp_path / "products", "run", [("BUNIT", "MJy/sr", "")],
+                                       [("NCOMBINE", n, "")], metrics, "ivm", psf,
+                                       [f"psf{k}.raw" for k in range(n)])
+    assert [f.name for f in files] == ["run_iter01_i2d.fits", "run_iter02_i2d.fits"]
+    raw = files[1].read_bytes()
+    hd = F.read_hdus(raw)
+    assert np.array_equal(F.image_data(raw, hd[1]), res.iterations[2])
+    assert hd[1].header["MUFRTOT"] == sum(r["n_total_frozen"] for r in res.freeze_rows if r["iteration"] == 2)
+
 
 @pytest.mark.gpu
 def test_dropin_freeze_forcing(small_ds, tmp_path):
