@@ -480,7 +480,8 @@ def main() -> int:
     # roofline of the dominant kernel
     e, c, ed, ea = 4, 8, 8, 8
     sb = metrics.stage_bytes(ds.W, ds.H, FW, FH, [i["nin"] for i in infos], [i["nlinks"] for i in infos],
-                             [i["ncsc"] for i in infos], [i["linkmap_bytes"] for i in infos], e, c, ed, ea)
+                             [i["ncsc"] for i in infos], [i["linkmap_bytes"] for i in infos], e, c, ed, ea,
+                             fused_inverse="col_inv_corr" not in prof)
     kern = {k: v for k, v in prof.items() if k in sb}
     dom = max(kern, key=lambda k: kern[k][0])
     ms = kern[dom][0]

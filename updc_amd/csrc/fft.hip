@@ -354,7 +354,7 @@ template <class T, class TA, int NT, int VMAX, int XMAX, int MODE>
 __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__restrict__ in, size_t in_bs, int in_rp,
                                             Cx<T> *__restrict__ out, size_t out_bs, int out_rp,
                                             const Cx<T> *__restrict__ psf, size_t psf_bs, int psf_rp, int hin,
-                                            int hout, int nexp) {
+                                            int hout, int nexp, FftLen fa) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
     const int col = xcd_tile(blockIdx.x, nc);
@@ -423,8 +423,23 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
             }
         }
         if (MODE == COL_ACC) {
-            // the sum itself, in TA, for an inverse transform in TA
             Cx<TA> *o = reinterpret_cast<Cx<TA> *>(out) + (size_t)col * out_rp;
+            if (fa.L > 0) {
+                // the inverse of the sum in TA (plan fa, TA twiddles) over the
+                // whole LDS allocation, FH TA values
+                Cx<TA> *bufa = reinterpret_cast<Cx<TA> *>(smem_raw);
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < XMAX; ++j) {
+                    const int e = tid + j * NT;
+                    if (e < FH) bufa[e] = acc[j];
+                }
+                __syncthreads();
+                fft_lds<true, TA, NT, VMAX>(bufa, fa, 0);
+                for (int n = tid; n < hout; n += NT) o[n] = bufa[n];
+                return;
+            }
+            // the sum itself, in TA, for an inverse transform in TA
 #pragma unroll
             for (int j = 0; j < XMAX; ++j) {
                 const int e = tid + j * NT;
@@ -462,7 +477,7 @@ template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF, int EV
 __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__restrict__ in, size_t in_bs, int in_rp,
                                               Cx<T> *__restrict__ out, size_t out_bs, int out_rp,
                                               const Cx<T> *__restrict__ psf, size_t psf_bs, int psf_rp, int hin,
-                                              int hout, int nexp) {
+                                              int hout, int nexp, FftLen fa) {
     constexpr int NBE = EV / RE;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
@@ -553,8 +568,18 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
                 }
         }
         if (MODE == COL_ACC) {
-            // the sum itself, in TA, for an inverse transform in TA
             Cx<TA> *o = reinterpret_cast<Cx<TA> *>(out) + (size_t)col * out_rp;
+            if (fa.L > 0) {
+                // the inverse of the sum in TA (plan fa: the same radices, TA
+                // twiddles) over the whole LDS allocation, FH TA values; the
+                // last forward transform's edge stage ended with a barrier
+                Cx<TA> *bufa = reinterpret_cast<Cx<TA> *>(smem_raw);
+                const Cx<TA> *twla = (const Cx<TA> *)fa.tw + fa.to[ns - 1];
+                edge_first<RE, true, TA, NT, NBE>(acc, bufa, LR);
+                fft_lds_range<true, TA, NT, VMAX>(bufa, fa, 1, ns - 1);
+                edge_last<RE, true, TA, NT, NBE>(acc, bufa, LR, twla);
+            }
+            // the sum (or its inverse), in TA
 #pragma unroll
             for (int r = 0; r < NBE; ++r)
 #pragma unroll
@@ -836,8 +861,11 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
                 return "cannot raise LDS limit for the column pass";
     }
     if constexpr (sizeof(T) < sizeof(double)) {
+        // the accumulate pass may run its FP64 inverse over FH double values
         void *fn_acc = col_re ? col_acc_e_fn<T, double>(col_re) : col_fn<T, double, COL_ACC>();
-        if (!fn_acc || hipFuncSetAttribute(fn_acc, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
+        const int acc_lds = (int)std::min<size_t>((size_t)FH * sizeof(Cx<double>), LDS_BUDGET);
+        if (!fn_acc || hipFuncSetAttribute(fn_acc, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           std::max(col_lds, acc_lds)) != hipSuccess)
             return "cannot raise LDS limit for the column pass";
     }
     return "";
@@ -897,9 +925,18 @@ template <class TA>
 hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
                             size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
                             int hout, int batch, hipStream_t st) {
+    FftLen none{};  // no TA inverse
+    return colpass_impl<TA>(mode, in, in_bstride, in_rows, out, out_bstride, out_rows, psf, psf_bstride, hin, hout,
+                            batch, none, (size_t)s.FH * sizeof(Cx<T>), st);
+}
+
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
+                                 size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
+                                 int hout, int batch, FftLen fa, size_t lds, hipStream_t st) {
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
     const int gx = (s.NC + 7) / 8 * 8;
-    const size_t lds = (size_t)s.FH * sizeof(Cx<T>);
     hin = std::min(hin, s.FH);
     hout = std::min(hout, s.FH);
     const int irp = rp(in_rows), orp = rp(out_rows), prp = rp(s.FH);
@@ -915,7 +952,7 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in
         Cx<T> *pout = out;
         void *args[] = {(void *)&col, (void *)&nc, (void *)&pin, (void *)&in_bstride, (void *)&irp,
                         (void *)&pout, (void *)&out_bstride, (void *)&orp, (void *)&ppsf, (void *)&psf_bstride,
-                        (void *)&prp, (void *)&hin, (void *)&hout, (void *)&nexp};
+                        (void *)&prp, (void *)&hin, (void *)&hout, (void *)&nexp, (void *)&fa};
         const dim3 grid(gx, (mode == COL_FWD || mode == COL_INV) ? batch : 1);
         return hipLaunchKernel(fn, grid, dim3(CNT), args, lds, st);
     }
@@ -923,26 +960,28 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in
         case COL_FWD:
             hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_FWD>), dim3(gx, batch), dim3(CNT), lds, st, col,
                                s.NC, in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout,
-                               batch);
+                               batch, fa);
             break;
         case COL_INV:
             hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_INV>), dim3(gx, batch), dim3(CNT), lds, st, col,
                                s.NC, in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout,
-                               batch);
+                               batch, fa);
             break;
         case COL_BCAST:
             hipLaunchKernelGGL((k_col<T, T, CNT, CVMAX, CXMAX, COL_BCAST>), dim3(gx, 1), dim3(CNT), lds, st, col, s.NC,
-                               in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch);
+                               in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch,
+                               fa);
             break;
         case COL_ADJ:
             hipLaunchKernelGGL((k_col<T, TA, CNT, CVMAX, CXMAX, COL_ADJ>), dim3(gx, 1), dim3(CNT), lds, st, col, s.NC,
-                               in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch);
+                               in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout, batch,
+                               fa);
             break;
         default:
             if constexpr (sizeof(TA) > sizeof(T)) {
                 hipLaunchKernelGGL((k_col<T, TA, CNT, CVMAX, CXMAX, COL_ACC>), dim3(gx, 1), dim3(CNT), lds, st, col,
                                    s.NC, in, in_bstride, irp, out, out_bstride, orp, psf, psf_bstride, prp, hin, hout,
-                                   batch);
+                                   batch, fa);
             } else {
                 return hipErrorInvalidValue;
             }
@@ -951,13 +990,29 @@ hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in
     return hipGetLastError();
 }
 
+template <class T> bool Fft2<T>::acc_inverse_ok(const FftLen &fa) const {
+    constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX;
+    if (fa.L != s.FH || (size_t)s.FH * sizeof(Cx<double>) > LDS_BUDGET || !fits(fa, 1, CNT, CVMAX)) return false;
+    if (!col_re) return true;  // the LDS-only kernel runs plan fa as it is
+    // the edge kernel keeps its own edge radix: plan fa must have the same radices
+    if (fa.nst != col.nst) return false;
+    for (int i = 0; i < col.nst; ++i)
+        if (fa.rad[i] != col.rad[i]) return false;
+    return true;
+}
+
 template <class T>
 template <class TA>
-hipError_t Fft2<T>::colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, const Cx<T> *psf,
-                           size_t psf_bstride, int hin, int batch, hipStream_t st) {
-    // the kernels write Cx<TA> elements at pitch rp(FH) through `out`
-    return colpass<TA>(COL_ACC, in, in_bstride, in_rows, reinterpret_cast<Cx<T> *>(out), 0, s.FH, psf, psf_bstride,
-                       hin, s.FH, batch, st);
+hipError_t Fft2<T>::colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, int out_rows, int hout,
+                           const Cx<T> *psf, size_t psf_bstride, int hin, int batch, const FftLen *fa,
+                           hipStream_t st) {
+    // the kernels write Cx<TA> elements at pitch rp(out_rows) through `out`
+    if (fa && !acc_inverse_ok(*fa)) return hipErrorInvalidValue;
+    FftLen none{};
+    const size_t lds = (size_t)s.FH * (fa ? sizeof(Cx<TA>) : sizeof(Cx<T>));
+    return colpass_impl<TA>(COL_ACC, in, in_bstride, in_rows, reinterpret_cast<Cx<T> *>(out), 0,
+                            fa ? out_rows : s.FH, psf, psf_bstride, hin, fa ? hout : s.FH, batch, fa ? *fa : none,
+                            lds, st);
 }
 
 template <class T> hipError_t Fft2<T>::spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st) {
@@ -997,7 +1052,7 @@ template hipError_t Fft2<float>::colpass<double>(int, const Cx<float> *, size_t,
                                                  const Cx<float> *, size_t, int, int, int, hipStream_t);
 template hipError_t Fft2<double>::colpass<double>(int, const Cx<double> *, size_t, int, Cx<double> *, size_t, int,
                                                   const Cx<double> *, size_t, int, int, int, hipStream_t);
-template hipError_t Fft2<float>::colacc<double>(const Cx<float> *, size_t, int, Cx<double> *, const Cx<float> *,
-                                                size_t, int, int, hipStream_t);
+template hipError_t Fft2<float>::colacc<double>(const Cx<float> *, size_t, int, Cx<double> *, int, int,
+                                                const Cx<float> *, size_t, int, int, const FftLen *, hipStream_t);
 
 }  // namespace updc

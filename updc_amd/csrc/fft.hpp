@@ -62,11 +62,21 @@ template <class T> struct Fft2 {
     hipError_t colpass(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out, size_t out_bstride,
                        int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin, int hout, int batch,
                        hipStream_t st);
-    // COL_ACC: out = sum_k conj(psf[k]) * FFT(in[k]) kept in TA (TA wider than
-    // T), all FH rows, pitch rp(FH), for an inverse transform in TA.
+    // COL_ACC: A = sum_k conj(psf[k]) * FFT(in[k]) kept in TA (TA wider than
+    // T).  fa == nullptr: out = A, all FH rows, pitch rp(FH).  Else out =
+    // IFFT(A) computed in TA inside the same pass with plan fa (the TA plan of
+    // the same height, accepted by acc_inverse_ok), rows [0, hout), pitch
+    // rp(out_rows).
     template <class TA>
-    hipError_t colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, const Cx<T> *psf,
-                      size_t psf_bstride, int hin, int batch, hipStream_t st);
+    hipError_t colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, int out_rows, int hout,
+                      const Cx<T> *psf, size_t psf_bstride, int hin, int batch, const FftLen *fa, hipStream_t st);
+    // Whether colacc can run the TA inverse with plan fa (same radices as the
+    // edge kernel's plan, registers and LDS within the column configuration).
+    bool acc_inverse_ok(const FftLen &fa) const;
+    template <class TA>
+    hipError_t colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
+                            size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
+                            int hout, int batch, FftLen fa, size_t lds, hipStream_t st);
     // Plain (per-exposure) form helpers over full column-major spectra (pitch rp(FH)):
     //   out = a * psf_k ;  acc += conj(psf_k) * a  (acc in TA) ;  out = (T) acc
     hipError_t spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st);

@@ -309,7 +309,8 @@ template <class TF, class TA> struct Pipe : PipeBase {
     DevBuf<uint8_t> frozen;
     DevBuf<Cx<TF>> psf_spec, spec_x, spec_k, spec_tmp;
     DevBuf<Cx<TA>> spec_acc;  // summed adjoint spectrum (per-exposure form; kWide: every form), pitch rp(FH)
-    DevBuf<Cx<TA>> spec_xa;   // kWide: spec_acc after the TA column inverse, rows [0, H)
+    DevBuf<Cx<TA>> spec_xa;   // kWide: the sum after its TA column inverse, rows [0, H)
+    bool acc_fused = false;   // kWide: that inverse runs inside the accumulating column pass
     DevBuf<ExpDiag> diag_cur, diag_hist;
     DevBuf<IterDiag> idiag_cur, idiag_hist;
     DevBuf<double> partials;
@@ -337,10 +338,17 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const FftShape &s = fft.s;
         const size_t kstride = fft.spec_elems(c.H), fstride = fft.spec_elems(s.FH);
         if constexpr (kWide) {
-            HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_acc.p, psf_spec.p, fstride, c.H, n, c.st));
-            mark();
-            HIPCHK(fa.template colpass<TA>(COL_INV, spec_acc.p, 0, s.FH, spec_xa.p, 0, c.H, nullptr, 0, s.FH, c.H, 1,
-                                           c.st));
+            if (acc_fused) {
+                // the column inverse of the sum runs in TA inside the accumulating pass
+                HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_xa.p, c.H, c.H, psf_spec.p, fstride, c.H,
+                                               n, &fa.col, c.st));
+            } else {
+                HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_acc.p, s.FH, s.FH, psf_spec.p, fstride,
+                                               c.H, n, nullptr, c.st));
+                mark();
+                HIPCHK(fa.template colpass<TA>(COL_INV, spec_acc.p, 0, s.FH, spec_xa.p, 0, c.H, nullptr, 0, s.FH, c.H,
+                                               1, c.st));
+            }
             mark();
             HIPCHK(fa.row_c2r(spec_xa.p, 0, c.H, C.p, 0, c.W, c.W, scale_a(c), c.H, 1, c.st));
         } else {
@@ -623,7 +631,8 @@ template <class TF, class TA> struct Pipe : PipeBase {
         spec_x.alloc(fft.spec_elems(std::max(c.H, s.FH)));
         spec_k.alloc((size_t)n * fft.spec_elems(c.H));
         if (kWide) {
-            spec_acc.alloc(fft.spec_elems(s.FH));
+            acc_fused = fft.acc_inverse_ok(fa.col);
+            if (!acc_fused) spec_acc.alloc(fft.spec_elems(s.FH));
             spec_xa.alloc(fa.spec_elems(c.H));
         }
         HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, npix, c.st));
@@ -797,7 +806,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             if (split) stage_names.insert(stage_names.end(), {"adjoint_image", "row_r2c_adj"});
             else stage_names.push_back("row_r2c_gather");
             stage_names.push_back("col_adj_accum");
-            if (kWide) stage_names.push_back("col_inv_corr");
+            if (kWide && !acc_fused) stage_names.push_back("col_inv_corr");
             stage_names.insert(stage_names.end(), {"row_c2r_corr", "update"});
         }
         const int nst = (int)stage_names.size();

@@ -25,14 +25,16 @@ def stats(ref: np.ndarray, new: np.ndarray) -> dict:
 
 
 def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[int], ncsc: list[int],
-                lm_bytes: list[int], e: int, c: int, e_d: int, e_a: int | None = None) -> dict:
+                lm_bytes: list[int], e: int, c: int, e_d: int, e_a: int | None = None,
+                fused_inverse: bool = False) -> dict:
     """Algorithmic HBM bytes per launch of each iteration kernel: every
     operand read once and every result written once (FFT-internal LDS passes
     and re-reads excluded).  e/c = real/complex element bytes of the FFT
     type, e_d = bytes of the denominator element.  e_a: real element bytes of
     a summed adjoint spectrum wider than the FFT type (the mixed tier), which
     leaves the column pass whole (FH rows) and is inverted in that type by
-    col_inv_corr and row_c2r_corr."""
+    col_inv_corr and row_c2r_corr; with fused_inverse the column inverse runs
+    inside col_adj_accum, which then writes the H inverted rows."""
     N = len(nin)
     NC = FW // 2 + 1
     nout = W * H
@@ -61,7 +63,7 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
         acc_full = FH * NC * 2 * e_a
         acc_h = H * NC * 2 * e_a
         out.update({
-            "col_adj_accum": spec_k + psf + acc_full,
+            "col_adj_accum": spec_k + psf + (acc_h if fused_inverse else acc_full),
             "col_inv_corr": acc_full + acc_h,
             "row_c2r_corr": acc_h + nout * e_a,
             # x read and written, C in the wide type, D
