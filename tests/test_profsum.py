@@ -18,10 +18,11 @@ FUSED = [
     "void updc::k_col_e<double, double, 512, 16, 5, 1, false>(FftLen, ...)",
     "void updc::k_row_c2r_p<double, 512, 16, 2, 16>(FftLen, ...)",
     "void updc::k_update<float, double>(float*, ...)",
+    "void updc::k_row_c2r_upd_p<double, float, 1024, 10, 2, 6>(FftLen, ...)",
     "updc::k_commit_diag(ExpDiag const*, ...)",
 ]
 FUSED_STAGES = ["row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze", "row_r2c_gather", "col_adj_accum",
-                "col_inv_corr", "row_c2r_corr", "update"]
+                "col_inv_corr", "row_c2r_corr", "update", "row_c2r_update"]
 # the split form replaces the fused gather R2C by the adjoint image and a plain R2C
 SPLIT = FUSED[:4] + ["void updc::k_csc_image<float>(ExpDev const*, ...)",
                      "void updc::k_row_r2c_p<float, 1024, 16, 4, 4>(FftLen, ...)"] + FUSED[5:]

@@ -224,14 +224,14 @@ __device__ __forceinline__ void row_tiles(int ntiles, int &t, int &t_end, int &s
     t_end = min(ntiles, (xcd + 1) * tper);
 }
 
-// C2R (as k_row_c2r).  PFM complex values per thread cover (M + 1) * RB slots.
-template <class T, int NT, int VMAX, int RB, int PFM>
-__global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
-    FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in, size_t in_bstride, int RP,
-    T *__restrict__ out, size_t out_bstride, int out_pitch, int wout, T scale, int rows, int batch, int vec) {
+// C2R tiles of a persistent workgroup (as k_row_c2r).  PFM complex values per
+// thread cover (M + 1) * RB slots.  emit(b, r0, z) consumes each transformed
+// tile from LDS (z[(n << LG) + j] = columns 2n, 2n+1 of row r0 + j, unscaled).
+template <class T, int NT, int VMAX, int RB, int PFM, class Emit>
+__device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *__restrict__ post,
+                                              const Cx<T> *__restrict__ in, size_t in_bstride, int RP, int rows,
+                                              int batch, Cx<T> *z, Emit &emit) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = f.L, nblk = (rows + RB - 1) / RB, nsl = (M + 1) << LG;
     int t, t_end, step;
     row_tiles(nblk * batch, t, t_end, step);
@@ -276,15 +276,136 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
         }
         __syncthreads();
         fft_lds<true, T, NT, VMAX>(z, f, LG);
+        emit(b, r0, z);
+    }
+}
+
+template <class T, int NT, int VMAX, int RB, int PFM>
+__global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
+    FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in, size_t in_bstride, int RP,
+    T *__restrict__ out, size_t out_bstride, int out_pitch, int wout, T scale, int rows, int batch, int vec) {
+    constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
+    const int M = f.L;
+    auto emit = [&](int b, int r0, const Cx<T> *zz) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
             if (r0 + j >= rows) continue;
             T *orow = out + (size_t)b * out_bstride + (size_t)(r0 + j) * out_pitch;
             for (int n = threadIdx.x; n < M; n += NT) {
                 const int c = 2 * n;
-                const Cx<T> v = z[(n << LG) + j];
+                const Cx<T> v = zz[(n << LG) + j];
                 store_pair(orow, c, wout, v.re * scale, v.im * scale, vec != 0);
             }
+        }
+    };
+    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, post, in, in_bstride, RP, rows, batch, z, emit);
+}
+
+// Persistent C2R of the summed adjoint spectrum (batch 1) with the update
+// x <- x * max(C/D, 0) (as k_update in stages.hip: the same guards and the
+// same operations on each element, so x is bit-identical) applied where C
+// leaves the transform, instead of storing C and reading it back.  The
+// signed sum and the negative count: per-thread sums over the thread's
+// elements, then per workgroup, then the last workgroup to finish (ticket)
+// adds the workgroup partials in workgroup order.
+template <class T, class TF, int NT, int VMAX, int RB, int PFM>
+__global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
+    FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in, int RP, TF *__restrict__ x,
+    const T *__restrict__ D, int W, T scale, T max_sens, int rows, IterDiag *idiag, double *partials,
+    unsigned long long *negs, unsigned int *ticket) {
+    constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
+    const int M = f.L;
+    const T thr = sizeof(T) == 4 ? (T)1e-8f : (T)1e-8;
+    const T neg = sizeof(T) == 4 ? (T)(-1.0e-6f) : (T)(-1.0e-6);
+    double sum = 0.0;
+    unsigned long long nneg = 0, badidx = ~0ull;
+    unsigned int bad = 0;
+    auto upd = [&](size_t t, T c) __attribute__((always_inline)) {
+        const T d = D[t];
+        TF xv = x[t];
+        if (d > thr) {
+            T fct = c / d;
+            if (!isfinite(fct)) { bad |= UPDC_E_FACTOR; badidx = badidx < t ? badidx : t; }
+            if (fct < (T)0) {
+                if (c < neg * max_sens) { bad |= UPDC_E_NEGCORR; badidx = badidx < t ? badidx : t; }
+                fct = (T)0;
+            }
+            xv = (TF)((T)xv * fct);
+        } else {
+            xv = (TF)0;
+        }
+        if (!isfinite(xv)) { bad |= UPDC_E_ESTIMATE; badidx = badidx < t ? badidx : t; }
+        x[t] = xv;
+        sum += (double)xv;
+        nneg += xv < (TF)0 ? 1 : 0;
+    };
+    auto emit = [&](int, int r0, const Cx<T> *zz) __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            if (r0 + j >= rows) continue;
+            const size_t rowt = (size_t)(r0 + j) * W;
+            for (int n = threadIdx.x; n < M; n += NT) {
+                const int c = 2 * n;
+                const Cx<T> v = zz[(n << LG) + j];
+                if (c < W) upd(rowt + c, v.re * scale);
+                if (c + 1 < W) upd(rowt + c + 1, v.im * scale);
+            }
+        }
+    };
+    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, post, in, 0, RP, rows, 1, z, emit);
+
+    if (bad) {
+        atomicOr(&idiag->err, bad);
+        atomicMin(&idiag->err_index, badidx);
+    }
+    // workgroup reduction in LDS (the transform is done with z)
+    __syncthreads();
+    double *sh = reinterpret_cast<double *>(smem_raw);
+    unsigned long long *shn = reinterpret_cast<unsigned long long *>(sh + NT);
+    __shared__ bool last;
+    sh[threadIdx.x] = sum;
+    shn[threadIdx.x] = nneg;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sh[threadIdx.x] += sh[threadIdx.x + o];
+            shn[threadIdx.x] += shn[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = sh[0];
+        negs[blockIdx.x] = shn[0];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double s2 = 0.0;
+        unsigned long long n2 = 0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += NT) {
+            s2 += __builtin_nontemporal_load(&partials[i]);
+            n2 += __builtin_nontemporal_load(&negs[i]);
+        }
+        sh[threadIdx.x] = s2;
+        shn[threadIdx.x] = n2;
+        __syncthreads();
+        for (int o = NT / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                sh[threadIdx.x] += sh[threadIdx.x + o];
+                shn[threadIdx.x] += shn[threadIdx.x + o];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            idiag->signed_sum = sh[0];
+            idiag->n_negative = shn[0];
+            *ticket = 0u;
         }
     }
 }
@@ -772,6 +893,9 @@ template <class T> static int pairs_aligned(const void *p, int pitch, size_t bst
 template <class T> static void *row_c2r_p_fn() {
     return (void *)k_row_c2r_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM>;
 }
+template <class T, class TF> static void *row_c2r_upd_fn() {
+    return (void *)k_row_c2r_upd_p<T, TF, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM>;
+}
 template <class T> static void *row_r2c_p_fn() {
     return (void *)k_row_r2c_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFR>;
 }
@@ -833,7 +957,7 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     const int rlds = (int)row_lds<T>(s.M);
     const int col_lds = (int)((size_t)FH * sizeof(Cx<T>));
     void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_c2r_p_fn<T>(),
-                       row_r2c_p_fn<T>()};
+                       row_r2c_p_fn<T>(), row_c2r_upd_fn<T, float>(), row_c2r_upd_fn<T, T>()};
     // persistent row passes: one workgroup per CU, prefetch registers must cover a tile
     {
         int dev = 0;
@@ -917,6 +1041,19 @@ hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *
     const dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
     hipLaunchKernelGGL((k_row_c2r<T, NT, VMAX, RB>), grid, dim3(NT), lds, st, row, post,
                        in, in_bstride, rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows, vec);
+    return hipGetLastError();
+}
+
+template <class T>
+template <class TF>
+hipError_t Fft2<T>::row_c2r_update(const Cx<T> *in, int rp_rows, TF *x, const T *D, int W, int H, T scale,
+                                   T max_sens, IterDiag *idiag, double *partials, unsigned long long *negs,
+                                   unsigned int *ticket, hipStream_t st) {
+    if (!row_persistent) return hipErrorNotSupported;
+    constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
+    hipLaunchKernelGGL((k_row_c2r_upd_p<T, TF, NT, VMAX, RB, RowCfg<T>::PFM>), dim3(row_grid), dim3(NT),
+                       row_lds<T>(s.M), st, row, post, in, rp(rp_rows), x, D, W, scale, max_sens, H, idiag, partials,
+                       negs, ticket);
     return hipGetLastError();
 }
 
@@ -1040,6 +1177,15 @@ hipError_t Fft2<T>::spec_cast(const Cx<TA> *acc, Cx<T> *out, hipStream_t st) {
 
 template struct Fft2<float>;
 template struct Fft2<double>;
+template hipError_t Fft2<float>::row_c2r_update<float>(const Cx<float> *, int, float *, const float *, int, int,
+                                                       float, float, IterDiag *, double *, unsigned long long *,
+                                                       unsigned int *, hipStream_t);
+template hipError_t Fft2<double>::row_c2r_update<float>(const Cx<double> *, int, float *, const double *, int, int,
+                                                        double, double, IterDiag *, double *, unsigned long long *,
+                                                        unsigned int *, hipStream_t);
+template hipError_t Fft2<double>::row_c2r_update<double>(const Cx<double> *, int, double *, const double *, int,
+                                                         int, double, double, IterDiag *, double *,
+                                                         unsigned long long *, unsigned int *, hipStream_t);
 template hipError_t Fft2<float>::spec_acc<float>(const Cx<float> *, const Cx<float> *, Cx<float> *, hipStream_t);
 template hipError_t Fft2<float>::spec_acc<double>(const Cx<float> *, const Cx<float> *, Cx<double> *, hipStream_t);
 template hipError_t Fft2<double>::spec_acc<double>(const Cx<double> *, const Cx<double> *, Cx<double> *, hipStream_t);

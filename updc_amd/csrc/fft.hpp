@@ -52,6 +52,15 @@ template <class T> struct Fft2 {
     // rp(rp_rows)) -> out[b*out_bstride + r*out_pitch + c] for c < wout, times scale.
     hipError_t row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *out, size_t out_bstride, int out_pitch,
                        int wout, T scale, int rows, int batch, hipStream_t st);
+    // Row C2R of one spectrum (rows [0, H), pitch rp(rp_rows)) fused with the
+    // multiplicative update of x (W x H) by C = scale * C2R against D, the
+    // signed sum and negative count going to idiag (partials/negs: one slot
+    // per workgroup, ticket: zero between launches).  hipErrorNotSupported
+    // when the persistent row passes do not apply: run row_c2r and update.
+    template <class TF>
+    hipError_t row_c2r_update(const Cx<T> *in, int rp_rows, TF *x, const T *D, int W, int H, T scale, T max_sens,
+                              IterDiag *idiag, double *partials, unsigned long long *negs, unsigned int *ticket,
+                              hipStream_t st);
     // Column passes (complex, length FH) over column-major spectra.  Rows >=
     // hin read as zero; rows [0, hout) are stored.  Input pitch rp(in_rows),
     // output pitch rp(out_rows); PSF spectra have pitch rp(FH).

@@ -311,6 +311,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     DevBuf<Cx<TA>> spec_acc;  // summed adjoint spectrum (per-exposure form; kWide: every form), pitch rp(FH)
     DevBuf<Cx<TA>> spec_xa;   // kWide: the sum after its TA column inverse, rows [0, H)
     bool acc_fused = false;   // kWide: that inverse runs inside the accumulating column pass
+    bool fused_update = false;  // the final row C2R applies the update (persistent row pass available)
     DevBuf<ExpDiag> diag_cur, diag_hist;
     DevBuf<IterDiag> idiag_cur, idiag_hist;
     DevBuf<double> partials;
@@ -333,8 +334,10 @@ template <class TF, class TA> struct Pipe : PipeBase {
 
     // C = crop C2R(sum_k conj(P^_k) spec_k) / nf from the row spectra spec_k
     // of the n exposures (updc_core.c:985-995, 1280-1287).  Records one event
-    // after each launch but the last.
-    template <class Mark> void adjoint_sum(Ctx &c, int n, Mark mark) {
+    // after each launch but the last.  With update_x the final row C2R also
+    // applies the update to x (updc_core.c:1437-1457) where the persistent
+    // row pass allows it, and returns true; C is then not stored.
+    template <class Mark> bool adjoint_sum(Ctx &c, int n, Mark mark, bool update_x = false) {
         const FftShape &s = fft.s;
         const size_t kstride = fft.spec_elems(c.H), fstride = fft.spec_elems(s.FH);
         if constexpr (kWide) {
@@ -350,13 +353,25 @@ template <class TF, class TA> struct Pipe : PipeBase {
                                                1, c.st));
             }
             mark();
+            if (update_x && fused_update) {
+                HIPCHK(fa.template row_c2r_update<TF>(spec_xa.p, c.H, x.p, D.p, c.W, c.H, scale_a(c), (TA)max_sens,
+                                                      idiag_cur.p, partials.p, negs.p, ticket.p, c.st));
+                return true;
+            }
             HIPCHK(fa.row_c2r(spec_xa.p, 0, c.H, C.p, 0, c.W, c.W, scale_a(c), c.H, 1, c.st));
         } else {
             HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p, fstride,
                                             c.H, c.H, n, c.st));
             mark();
+            if (update_x && fused_update) {
+                HIPCHK(fft.template row_c2r_update<TF>(spec_x.p, c.H, x.p, (const TF *)D.p, c.W, c.H, scale(c),
+                                                       (TF)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p,
+                                                       c.st));
+                return true;
+            }
             HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
         }
+        return false;
     }
 
     // ---- PSF spectra P^_k for every exposure into `out` (column-major,
@@ -630,6 +645,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         C.alloc(nout);
         spec_x.alloc(fft.spec_elems(std::max(c.H, s.FH)));
         spec_k.alloc((size_t)n * fft.spec_elems(c.H));
+        fused_update = kWide ? fa.row_persistent : fft.row_persistent;
         if (kWide) {
             acc_fused = fft.acc_inverse_ok(fa.col);
             if (!acc_fused) spec_acc.alloc(fft.spec_elems(s.FH));
@@ -642,8 +658,9 @@ template <class TF, class TA> struct Pipe : PipeBase {
         hist_cap = 1024;
         diag_hist.alloc((size_t)hist_cap * n);
         idiag_hist.alloc(hist_cap);
-        partials.alloc(update_blocks());
-        negs.alloc(update_blocks());
+        const int nred = std::max({update_blocks(), fft.row_grid, fa.row_grid});  // reduction slots per launch
+        partials.alloc(nred);
+        negs.alloc(nred);
         ticket.alloc(1);
         ctr.alloc(1);
         HIPCHK(fill<unsigned int>(ticket.p, 0u, 1, c.st));
@@ -734,9 +751,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
             HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
         }
         mark(m++);
-        adjoint_sum(c, n, [&]() { mark(m++); });
-        mark(m++);
-        HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p, c.st)));
+        if (!adjoint_sum(c, n, [&]() { mark(m++); }, true)) {
+            mark(m++);
+            HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p,
+                                   c.st)));
+        }
         mark(m++);
         HIPCHK(commit_diag(diag_cur.p, n, idiag_cur.p, diag_hist.p, idiag_hist.p, hist_cap, ctr.p, c.st));
     }
@@ -807,7 +826,8 @@ template <class TF, class TA> struct Pipe : PipeBase {
             else stage_names.push_back("row_r2c_gather");
             stage_names.push_back("col_adj_accum");
             if (kWide && !acc_fused) stage_names.push_back("col_inv_corr");
-            stage_names.insert(stage_names.end(), {"row_c2r_corr", "update"});
+            if (fused_update) stage_names.push_back("row_c2r_update");
+            else stage_names.insert(stage_names.end(), {"row_c2r_corr", "update"});
         }
         const int nst = (int)stage_names.size();
         stage_ms.assign(nst + 1, 0.0);

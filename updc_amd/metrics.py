@@ -58,6 +58,8 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
         "col_adj_accum": spec_k + psf + spec_x,
         "row_c2r_corr": spec_x + nout * e,
         "update": nout * (3 * e + e_d),
+        # the final row C2R with the update applied: x in and out, D, no C
+        "row_c2r_update": spec_x + nout * (2 * e + e_d),
     }
     if e_a is not None and e_a > e:
         acc_full = FH * NC * 2 * e_a
@@ -68,5 +70,6 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
             "row_c2r_corr": acc_h + nout * e_a,
             # x read and written, C in the wide type, D
             "update": nout * (2 * e + e_a + e_d),
+            "row_c2r_update": acc_h + nout * (2 * e + e_d),
         })
     return out

@@ -31,7 +31,7 @@ from pathlib import Path
 
 # iteration stage names, in launch order (solver.hip launch_iteration)
 STAGES = ("row_r2c_x", "col_fwd_bcast", "row_c2r_blur", "gather_freeze", "row_r2c_gather", "adjoint_image",
-          "row_r2c_adj", "col_adj_accum", "col_inv_corr", "row_c2r_corr", "update")
+          "row_r2c_adj", "col_adj_accum", "col_inv_corr", "row_c2r_corr", "update", "row_c2r_update")
 
 
 def _template_args(name: str) -> list[str]:
@@ -52,6 +52,8 @@ def classify(name: str, prev: str | None) -> str | None:
         if _row_r2c_gather(name):
             return "row_r2c_gather"
         return "row_r2c_adj" if prev == "adjoint_image" else "row_r2c_x"
+    if "k_row_c2r_upd" in name:
+        return "row_c2r_update"
     if "k_row_c2r" in name:
         return "row_c2r_blur" if prev == "col_fwd_bcast" else "row_c2r_corr"
     if "k_col_e<" in name or "k_col<" in name:
