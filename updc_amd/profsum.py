@@ -120,7 +120,8 @@ def trace_summary(d: Path, cmds: list[str]) -> str:
             continue
         out.append(f"{st:16s} {len(v):5d} {statistics.fmean(v):9.1f} {min(v):9.1f} {max(v):9.1f} "
                    f"{100 * sum(v) / total:5.1f}%  {names[st]}")
-    n_it = len(per.get("update", [])) or 1
+    # every iteration opens with the row R2C of x
+    n_it = len(per.get("row_r2c_x", [])) or 1
     out += ["", f"iterations: {n_it}; iteration kernels: {total / n_it / 1e3:.3f} ms per iteration"]
     return "\n".join(out) + "\n"
 
