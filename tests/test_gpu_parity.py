@@ -373,3 +373,33 @@ def test_mixed_precision_budget():
     print(f"  norm_linf vs f64: f32 {e32:.3e}, mixed {emx:.3e}", flush=True)
     assert emx < 2e-6
     assert emx < 0.25 * e32
+
+
+def test_wide_rows_take_the_unfused_paths(small_ds):
+    """An FFT 6400 wide (rows of M = 3200 points) is beyond the persistent
+    row passes: the gather R2C, the row C2Rs and the update run as separate
+    one-tile-per-workgroup launches.  Against the FP64 oracle at the same
+    shape, every tier stays within its usual bound, and mixed agrees with its
+    own plain (per-exposure) form bit for bit."""
+    from oracle import oracle as O
+    from updc_amd._lib import FLAG_PER_EXPOSURE, check
+    shape = (6400, 112)
+    print(f"case: FFT {shape}, unfused row paths", flush=True)
+    ref = O.RefSolver(small_ds.W, small_ds.H, oracle_exposures(small_ds), "f64", fft_shape=shape)
+    ref.iterate(3)
+    xs = {}
+    for prec, tol in (("f64", 1e-11), ("mixed", 1e-4)):
+        c = _ctx(small_ds, prec, fft=shape)
+        c.prepare()
+        c.iterate(3)
+        xs[prec] = c.estimate(f64=True)
+        st = O.compare_stats(ref.x, xs[prec])
+        print(f"  {prec}: norm_linf {st['norm_linf']:.3e}", flush=True)
+        assert st["norm_linf"] < tol, (prec, st)
+        c.close()
+    p = _ctx(small_ds, "mixed", fft=shape)
+    check(p.lib.updc_ctx_set_flags(p.h, p.flags | FLAG_PER_EXPOSURE))
+    p.prepare()
+    p.iterate(3)
+    assert np.array_equal(p.estimate(f64=True), xs["mixed"])
+    p.close()
