@@ -589,49 +589,72 @@ __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__
     TA mn = (TA)INFINITY;
     const uint32_t step = gridDim.x * GATHER_NT;
     uint32_t p0 = blockIdx.x * GATHER_NT;
-    uint32_t Z0 = 0, Z1 = 0;
-    if (p0 < nin) {
-        Z0 = off[p0];
-        Z1 = off[min(p0 + (uint32_t)GATHER_NT, nin)];
-    }
-    for (; p0 < nin; p0 += step) {
-        const uint32_t p = p0 + threadIdx.x;
-        const uint32_t pend = min(p0 + (uint32_t)GATHER_NT, nin);
+    // Software pipeline over the block's chunks: while chunk i's blur gathers
+    // run, chunk i+1's staging blocks and pixel operands are in flight, and
+    // chunk i+2's link range is being read.
+    auto range = [&](uint32_t q0, uint32_t &A0, uint32_t &A1) {
+        if (q0 < nin) {
+            A0 = off[q0];
+            A1 = off[min(q0 + (uint32_t)GATHER_NT, nin)];
+        }
+    };
+    uint4 lv[GATHER_SLOTS], fv[GATHER_SLOTS];
+    // staging state of the chunk whose loads are in flight
+    uint32_t n_nl = 0, n_nf = 0, n_lsh = 0, n_fsh = 0, n_Z0 = 0, n_z0 = 0, n_z1 = 0, n_b0 = 0;
+    bool n_staged = false;
+    TF n_wp = (TF)0, n_sv = (TF)0;
+    uint8_t n_fz = 0;
+    auto stage = [&](uint32_t q0, uint32_t A0, uint32_t A1) __attribute__((always_inline)) {
         // aligned 16-byte block ranges covering the chunk's links and fractions
-        const uintptr_t la = lbase + (uintptr_t)Z0 * lsz, fa = (uintptr_t)(frac + Z0);
+        const uintptr_t la = lbase + (uintptr_t)A0 * lsz, fa = (uintptr_t)(frac + A0);
         const uintptr_t la0 = la & ~(uintptr_t)15, fa0 = fa & ~(uintptr_t)15;
-        const uint32_t nlv = (uint32_t)((((lbase + (uintptr_t)Z1 * lsz) + 15) & ~(uintptr_t)15) - la0) >> 4;
-        const uint32_t nfv = (uint32_t)((((uintptr_t)(frac + Z1) + 15) & ~(uintptr_t)15) - fa0) >> 4;
-        const bool staged = nlv <= (uint32_t)GATHER_LV && nfv <= (uint32_t)GATHER_FV;
-        const uint32_t lsh = (uint32_t)(la - la0) / lsz, fsh = (uint32_t)(fa - fa0) >> 2;
-        // round trip 1: staging blocks and the lane's pixel operands together
-        const uint32_t nl = staged ? nlv : 0u, nf = staged ? nfv : 0u;
-        uint4 lv[GATHER_SLOTS], fv[GATHER_SLOTS];
+        const uint32_t nlv = (uint32_t)((((lbase + (uintptr_t)A1 * lsz) + 15) & ~(uintptr_t)15) - la0) >> 4;
+        const uint32_t nfv = (uint32_t)((((uintptr_t)(frac + A1) + 15) & ~(uintptr_t)15) - fa0) >> 4;
+        n_staged = nlv <= (uint32_t)GATHER_LV && nfv <= (uint32_t)GATHER_FV;
+        n_lsh = (uint32_t)(la - la0) / lsz;
+        n_fsh = (uint32_t)(fa - fa0) >> 2;
+        n_Z0 = A0;
+        // the staging blocks and the lane's pixel operands, issued together
+        n_nl = n_staged ? nlv : 0u;
+        n_nf = n_staged ? nfv : 0u;
 #pragma unroll
         for (int s = 0; s < GATHER_SLOTS; ++s) {
             const uint32_t i = threadIdx.x + s * GATHER_NT;
-            lv[s] = i < nl ? ((const uint4 *)la0)[i] : uint4{};
-            fv[s] = i < nf ? ((const uint4 *)fa0)[i] : uint4{};
+            lv[s] = i < n_nl ? ((const uint4 *)la0)[i] : uint4{};
+            fv[s] = i < n_nf ? ((const uint4 *)fa0)[i] : uint4{};
         }
-        uint32_t z0 = 0, z1 = 0, b0 = 0;
-        TF wp = (TF)0, sv = (TF)0;
-        uint8_t fz = 0;
+        const uint32_t q = q0 + threadIdx.x;
+        const uint32_t qend = min(q0 + (uint32_t)GATHER_NT, nin);
+        n_z0 = n_z1 = n_b0 = 0;
+        n_wp = n_sv = (TF)0;
+        n_fz = 0;
+        if (q < qend) {
+            const size_t gq = e.pix + q;
+            n_z0 = off[q];
+            n_z1 = off[q + 1];
+            if (d16) n_b0 = e.base[q];
+            n_wp = w[gq];
+            n_sv = sci[gq];
+            n_fz = frozen[gq];
+        }
+    };
+    uint32_t Zn0 = 0, Zn1 = 0;
+    if (p0 < nin) {
+        uint32_t A0 = 0, A1 = 0;
+        range(p0, A0, A1);
+        stage(p0, A0, A1);
+    }
+    range(p0 + step, Zn0, Zn1);
+    for (; p0 < nin; p0 += step) {
+        const uint32_t p = p0 + threadIdx.x;
+        const uint32_t pend = min(p0 + (uint32_t)GATHER_NT, nin);
+        // this chunk's staging state
+        const bool staged = n_staged;
+        const uint32_t lsh = n_lsh, fsh = n_fsh, Z0 = n_Z0, nl = n_nl, nf = n_nf;
+        const uint32_t z0 = n_z0, z1 = n_z1, b0 = n_b0;
+        const TF wp = n_wp, sv = n_sv;
+        uint8_t fz = n_fz;
         const size_t gp = e.pix + p;
-        if (p < pend) {
-            z0 = off[p];
-            z1 = off[p + 1];
-            if (d16) b0 = e.base[p];
-            wp = w[gp];
-            sv = sci[gp];
-            fz = frozen[gp];
-        }
-        // the next chunk's link range
-        const uint32_t pn = p0 + step;
-        uint32_t Zn0 = 0, Zn1 = 0;
-        if (pn < nin) {
-            Zn0 = off[pn];
-            Zn1 = off[min(pn + (uint32_t)GATHER_NT, nin)];
-        }
         __syncthreads();  // previous chunk's LDS reads are done
 #pragma unroll
         for (int s = 0; s < GATHER_SLOTS; ++s) {
@@ -640,6 +663,10 @@ __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__
             if (i < nf) s_fv[i] = fv[s];
         }
         __syncthreads();
+        // the next chunk's loads, then the range of the one after
+        const uint32_t pn = p0 + step;
+        if (pn < nin) stage(pn, Zn0, Zn1);
+        range(pn + step, Zn0, Zn1);
         if (p < pend) {
             TF out = (TF)0;
             if (wp > (TF)0 && z0 != z1) {
@@ -696,8 +723,6 @@ __global__ __launch_bounds__(GATHER_NT) void k_gather(const ExpDev *__restrict__
             }
             wr[gp] = out;
         }
-        Z0 = Zn0;
-        Z1 = Zn1;
     }
     // block reduction, one atomic per block and quantity
     for (int o = 32; o >= 1; o >>= 1) {
