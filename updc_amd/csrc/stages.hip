@@ -1023,37 +1023,60 @@ hipError_t copy_ref(const void *src, void *dst, size_t bytes, hipStream_t st) {
 // Batched adjoint image: out[k * nout + t] = sum over the CSC entries of
 // target t of fma(frac, v[pix_k + p], .) in ascending p (the reference's
 // scatter order, updc_core.c:1171-1176), for every exposure k of the batch
-// (blockIdx.y).  Grid-stride over targets; the first CSCI_U entries of a
-// target and their v loads are issued together.
+// (blockIdx.y).  Grid-stride over chunks of CSCI_G * CSCI_NT targets; a lane
+// takes CSCI_G targets CSCI_NT apart (coalesced offset loads and stores) and
+// issues the first CSCI_U entries of all of them, then their v loads, together.
 static constexpr int CSCI_NT = 256, CSCI_U = 4;
+// 3 targets per lane (2 in FP64) keep the kernel at 62 VGPRs, 8 waves per SIMD:
+// measured at config 2, 0.94 ms against 1.00 (1 target), 1.13 (4) and 1.29 (6).
+template <class TF> constexpr int csci_g() { return sizeof(TF) == 4 ? 3 : 2; }
 template <class TF>
 __global__ __launch_bounds__(CSCI_NT) void k_csc_image(const ExpDev *__restrict__ ex, const TF *__restrict__ v,
                                                        size_t nout, TF *__restrict__ out) {
+    constexpr int G = csci_g<TF>();
     const ExpDev &e = ex[blockIdx.y];
     const TF *vk = v + e.pix;
     TF *ok = out + (size_t)blockIdx.y * nout;
-    for (size_t t = (size_t)blockIdx.x * CSCI_NT + threadIdx.x; t < nout; t += (size_t)gridDim.x * CSCI_NT) {
-        const uint32_t a = e.toff[t], n = e.toff[t + 1] - a;
-        uint2 en[CSCI_U];
-        TF wv[CSCI_U];
+    const size_t step = (size_t)gridDim.x * CSCI_NT * G;
+    for (size_t t0 = (size_t)blockIdx.x * CSCI_NT * G + threadIdx.x; t0 < nout; t0 += step) {
+        uint32_t a[G], n[G];
 #pragma unroll
-        for (int j = 0; j < CSCI_U; ++j) en[j] = (uint32_t)j < n ? e.tent[a + j] : make_uint2(0u, 0u);
-#pragma unroll
-        for (int j = 0; j < CSCI_U; ++j) wv[j] = (uint32_t)j < n ? vk[en[j].x] : (TF)0;
-        TF acc = (TF)0;
-#pragma unroll
-        for (int j = 0; j < CSCI_U; ++j)
-            if ((uint32_t)j < n) acc = fmat((TF)__uint_as_float(en[j].y), wv[j], acc);
-        for (uint32_t j = CSCI_U; j < n; ++j) {
-            const uint2 x = e.tent[a + j];
-            acc = fmat((TF)__uint_as_float(x.y), vk[x.x], acc);
+        for (int g = 0; g < G; ++g) {
+            const size_t t = t0 + (size_t)g * CSCI_NT;
+            a[g] = t < nout ? e.toff[t] : 0u;
+            n[g] = t < nout ? e.toff[t + 1] : 0u;
         }
-        ok[t] = acc;
+        uint2 en[G][CSCI_U];
+        TF wv[G][CSCI_U];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            n[g] -= a[g];
+#pragma unroll
+            for (int j = 0; j < CSCI_U; ++j) en[g][j] = (uint32_t)j < n[g] ? e.tent[a[g] + j] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int j = 0; j < CSCI_U; ++j) wv[g][j] = (uint32_t)j < n[g] ? vk[en[g][j].x] : (TF)0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const size_t t = t0 + (size_t)g * CSCI_NT;
+            TF acc = (TF)0;
+#pragma unroll
+            for (int j = 0; j < CSCI_U; ++j)
+                if ((uint32_t)j < n[g]) acc = fmat((TF)__uint_as_float(en[g][j].y), wv[g][j], acc);
+            for (uint32_t j = CSCI_U; j < n[g]; ++j) {
+                const uint2 x = e.tent[a[g] + j];
+                acc = fmat((TF)__uint_as_float(x.y), vk[x.x], acc);
+            }
+            if (t < nout) ok[t] = acc;
+        }
     }
 }
 template <class TF> hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, hipStream_t st) {
     if (nexp <= 0) return hipSuccess;
-    const unsigned gx = (unsigned)std::min<size_t>((nout + CSCI_NT - 1) / CSCI_NT, (size_t)std::max(1, 16384 / nexp));
+    const size_t chunk = (size_t)CSCI_NT * csci_g<TF>();
+    const unsigned gx = (unsigned)std::min<size_t>((nout + chunk - 1) / chunk, (size_t)std::max(1, 16384 / nexp));
     hipLaunchKernelGGL((k_csc_image<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
     return hipGetLastError();
 }
