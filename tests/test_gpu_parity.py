@@ -403,3 +403,31 @@ def test_wide_rows_take_the_unfused_paths(small_ds):
     p.iterate(3)
     assert np.array_equal(p.estimate(f64=True), xs["mixed"])
     p.close()
+
+
+def test_widest_row_fft_and_the_limit_beyond(small_ds):
+    """Rows of M = 4096 points (FFT 8192 wide, radix 8 only) are the widest a
+    4-row tile's registers hold (two radix-8 butterflies per thread per
+    stage): every tier matches the FP64 oracle at that shape.  M = 4200
+    (FFT 8400, a radix-7 stage) needs three radix-7 butterflies per thread, and
+    the context refuses the shape with an error naming the row pass instead of
+    running anything."""
+    from oracle import oracle as O
+    from updc_amd._lib import UpdcError
+    shape = (8192, 112)
+    print(f"case: FFT {shape}, widest rows", flush=True)
+    ref = O.RefSolver(small_ds.W, small_ds.H, oracle_exposures(small_ds), "f64", fft_shape=shape)
+    ref.iterate(2)
+    for prec, tol in (("f64", 1e-11), ("mixed", 1e-4), ("f32", 1e-4)):
+        c = _ctx(small_ds, prec, fft=shape)
+        c.prepare()
+        c.iterate(2)
+        st = O.compare_stats(ref.x, c.estimate(f64=True))
+        print(f"  {prec}: norm_linf {st['norm_linf']:.3e}", flush=True)
+        assert st["norm_linf"] < tol, (prec, st)
+        c.close()
+    print("case: FFT (8400, 112) refused", flush=True)
+    c = _ctx(small_ds, "f32", fft=(8400, 112))
+    with pytest.raises(UpdcError, match="FFT width too large"):
+        c.prepare()
+    c.close()
