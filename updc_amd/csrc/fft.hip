@@ -371,7 +371,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
     shn[threadIdx.x] = nneg;
     __syncthreads();
     for (int o = NT / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
+        if ((int)threadIdx.x < o) {
             sh[threadIdx.x] += sh[threadIdx.x + o];
             shn[threadIdx.x] += shn[threadIdx.x + o];
         }
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
         shn[threadIdx.x] = n2;
         __syncthreads();
         for (int o = NT / 2; o > 0; o >>= 1) {
-            if (threadIdx.x < o) {
+            if ((int)threadIdx.x < o) {
                 sh[threadIdx.x] += sh[threadIdx.x + o];
                 shn[threadIdx.x] += shn[threadIdx.x + o];
             }

@@ -1008,7 +1008,6 @@ template <class TF, class TA> struct Pipe : PipeBase {
     }
     void stage_adjoint(Ctx &c, const double *v, double *out) override {
         const int n = N(c);
-        const FftShape &s = fft.s;
         const size_t kstride = fft.spec_elems(c.H);
         from_host_f64(v, c.npix(), wr.p, c);
         HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));

@@ -840,7 +840,7 @@ __global__ __launch_bounds__(UPD_NT) void k_update(TF *__restrict__ x, const TA 
     shn[threadIdx.x] = nneg;
     __syncthreads();
     for (int o = UPD_NT / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
+        if ((int)threadIdx.x < o) {
             sh[threadIdx.x] += sh[threadIdx.x + o];
             shn[threadIdx.x] += shn[threadIdx.x + o];
         }
@@ -865,7 +865,7 @@ __global__ __launch_bounds__(UPD_NT) void k_update(TF *__restrict__ x, const TA 
         shn[threadIdx.x] = n;
         __syncthreads();
         for (int o = UPD_NT / 2; o > 0; o >>= 1) {
-            if (threadIdx.x < o) {
+            if ((int)threadIdx.x < o) {
                 sh[threadIdx.x] += sh[threadIdx.x + o];
                 shn[threadIdx.x] += shn[threadIdx.x + o];
             }
