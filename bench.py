@@ -431,12 +431,12 @@ def main() -> int:
     its32 = K / dt32
     log(f"FP32/batched (mixed, graph): {K} iterations in {dt32 * 1e3:.1f} ms -> {its32:.1f} it/s")
     x32 = fast.estimate(f64=True)
-    # the adjoint gather as its own launch (FLAG_SPLIT_ADJOINT): timed beside the fused form
-    fast.set_split_adjoint(True)
-    dts, _, _ = timed_run(fast, K, Wm)
-    xs = fast.estimate(f64=True)
+    # the adjoint gather fused into the row R2C (FLAG_FUSED_ADJOINT): timed beside the default split form
     fast.set_split_adjoint(False)
-    log(f"split adjoint (graph): {K / dts:.1f} it/s; equal to fused: {bool(np.array_equal(xs, x32))}")
+    dtf, _, _ = timed_run(fast, K, Wm)
+    xf = fast.estimate(f64=True)
+    fast.set_split_adjoint(True)
+    log(f"fused adjoint (graph): {K / dtf:.1f} it/s; equal to split: {bool(np.array_equal(xf, x32))}")
     if args.profile:
         print(json.dumps(dict(metric=METRIC, value=its32, unit="iters/s", steps=K, warmup=Wm)))
         return 0

@@ -52,9 +52,10 @@ extern "C" {
 #define UPDC_FLAG_PER_EXPOSURE 4u /* one launch per exposure and stage (no batching) */
 #define UPDC_FLAG_AUTOTUNE 8u   /* time the candidate FFT shapes on the GPU at prepare
                                    (when the context was created with a 0 x 0 shape) */
-#define UPDC_FLAG_SPLIT_ADJOINT 16u /* adjoint gather as its own launch (all exposures'
-                                       S_k^T wr_k images), then a plain row R2C, instead of
-                                       the gather fused into the row R2C */
+#define UPDC_FLAG_FUSED_ADJOINT 16u /* batched form: the adjoint gather fused into the row
+                                       R2C (the S_k^T wr_k images are never stored), instead
+                                       of the default: the gather as its own launch for all
+                                       exposures, then a plain row R2C of the images */
 
 /* ---- LinkMap ABI (updc/cache_abi.py:9-12, updc_core.c:44-54) -------------- */
 #define UPDC_LINKMAP_MAGIC 0x324D4C55u

@@ -20,7 +20,7 @@ FLAG_GRAPH = 1
 FLAG_QUIET = 2
 FLAG_PER_EXPOSURE = 4
 FLAG_AUTOTUNE = 8
-FLAG_SPLIT_ADJOINT = 16
+FLAG_FUSED_ADJOINT = 16
 
 
 class UpdcError(RuntimeError):

@@ -740,9 +740,12 @@ template <class TF, class TA> struct Pipe : PipeBase {
                                       idiag_cur.p, c.st)));
         mark(4);
         int m = 5;
-        if (c.flags & UPDC_FLAG_SPLIT_ADJOINT) {
-            // S_k^T wr_k for every exposure into the (now dead) blur images, then
-            // the plain row R2C of those images
+        if (!(c.flags & UPDC_FLAG_FUSED_ADJOINT)) {
+            // the default: S_k^T wr_k for every exposure into the (now dead) blur
+            // images, then the plain row R2C of those images.  The standalone
+            // gather runs at 8 waves per SIMD; fused into the row R2C it shares
+            // the CU with the transform tile at 4, and the fused launch is slower
+            // than the two (1.58 ms against 0.91 + 0.60 ms at config 2)
             HIPCHK(csc_image<TF>(d_ex.p, n, wr.p, nout, blur.p, c.st));
             mark(m++);
             HIPCHK(fft.row_r2c(ROW_PLAIN, blur.p, nout, c.W, c.W, nullptr, nullptr, 0, spec_k.p, kstride, c.H, c.H, n,
@@ -817,7 +820,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         float total_ms = 0;
         HIPCHK(hipEventElapsedTime(&total_ms, it0, it1));
         const bool plain = (c.flags & UPDC_FLAG_PER_EXPOSURE) != 0;
-        const bool split = (c.flags & UPDC_FLAG_SPLIT_ADJOINT) != 0;
+        const bool split = (c.flags & UPDC_FLAG_FUSED_ADJOINT) == 0;
         if (plain)
             stage_names = {"plain_xhat", "plain_exposures", "plain_adjoint_inverse", "update"};
         else {
@@ -1495,7 +1498,7 @@ int updc_ctx_destroy(updc_ctx *ctx) {
 int updc_ctx_set_flags(updc_ctx *ctx, unsigned flags) {
     return guarded([&] {
         need(ctx != nullptr, UPDC_ERR_ARG, "null context");
-        const unsigned shape = UPDC_FLAG_PER_EXPOSURE | UPDC_FLAG_SPLIT_ADJOINT;
+        const unsigned shape = UPDC_FLAG_PER_EXPOSURE | UPDC_FLAG_FUSED_ADJOINT;
         if (((ctx->flags ^ flags) & shape) && ctx->pipe) ctx->pipe->invalidate_graph();
         ctx->flags = flags;
     });

@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (FLAG_AUTOTUNE, FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FLAG_SPLIT_ADJOINT, FreezeRow,
+from ._lib import (FLAG_AUTOTUNE, FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FLAG_FUSED_ADJOINT, FreezeRow,
                    IterInfo, check)
 
 
@@ -121,7 +121,7 @@ class Context:
     def set_split_adjoint(self, on: bool) -> None:
         """Adjoint gather as its own launch followed by a plain row R2C (True),
         or fused into the row R2C (False)."""
-        self.flags = (self.flags | FLAG_SPLIT_ADJOINT) if on else (self.flags & ~FLAG_SPLIT_ADJOINT)
+        self.flags = (self.flags & ~FLAG_FUSED_ADJOINT) if on else (self.flags | FLAG_FUSED_ADJOINT)
         check(self.lib.updc_ctx_set_flags(self.h, self.flags))
 
     def set_graph(self, on: bool) -> None:
