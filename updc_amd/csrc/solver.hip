@@ -745,7 +745,8 @@ template <class TF, class TA> struct Pipe : PipeBase {
             // images, then the plain row R2C of those images.  The standalone
             // gather runs at 8 waves per SIMD; fused into the row R2C it shares
             // the CU with the transform tile at 4, and the fused launch is slower
-            // than the two (1.58 ms against 0.91 + 0.60 ms at config 2)
+            // than the two (1.58 ms against 0.91 + 0.60 ms with 8 exposures of
+            // 2048² on a 4096² grid)
             HIPCHK(csc_image<TF>(d_ex.p, n, wr.p, nout, blur.p, c.st));
             mark(m++);
             HIPCHK(fft.row_r2c(ROW_PLAIN, blur.p, nout, c.W, c.W, nullptr, nullptr, 0, spec_k.p, kstride, c.H, c.H, n,
