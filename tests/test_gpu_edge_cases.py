@@ -1,0 +1,72 @@
+"""Empty inputs on the GPU path, against the FP64 oracle.
+
+- An exposure whose footprint lies entirely outside the target grid: its
+  LinkMap has no links, so none of its pixels is active (updc_core.c:1153, a
+  pixel needs w > 0 and links), its CSC is empty, and it adds nothing to the
+  adjoint or the denominator.
+- An exposure whose weights are all zero (fully masked): every pixel is
+  inactive, and the weight scale comes from the other exposure
+  (updc_core.c:378-424).
+
+In both cases the remaining exposure must drive the solve exactly as the
+oracle's does: FP64 within 1e-11, FreezeMu rows equal, the empty exposure's
+rows all zero.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_dataset, oracle_exposures
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(ds, precision, iters=3):
+    from updc_amd.solver import Context, Exposure
+    c = Context(ds.W, ds.H, precision)
+    for k in range(len(ds.geoms)):
+        c.add_exposure(Exposure(ds.sci[k], ds.wht[k], ds.psfs[k], ds.background[k], ds.pedestal[k],
+                                vertices=ds.vertices(k)))
+    links = [c.linkmap(k) for k in range(len(ds.geoms))]
+    c.prepare()
+    rows, _ = c.iterate(iters)
+    x = c.estimate(f64=True)
+    c.close()
+    return x, rows, links
+
+
+def _check(ds, empty_k):
+    from oracle import oracle as O
+    ref = O.RefSolver(ds.W, ds.H, oracle_exposures(ds), "f64")
+    rows_o, _ = ref.iterate(3)
+    x, rows, links = _solve(ds, "f64")
+    assert links == ds.linkmaps
+    st = O.compare_stats(ref.x, x)
+    print(f"  f64: norm_linf {st['norm_linf']:.3e}", flush=True)
+    assert st["norm_linf"] < 1e-11, st
+    assert [r[:4] for r in rows] == [r[:4] for r in rows_o]
+    assert all(r[2] == 0 and r[3] == 0 for r in rows if r[1] == empty_k + 1)
+    xm, _, _ = _solve(ds, "mixed")
+    stm = O.compare_stats(ref.x, xm)
+    print(f"  mixed: norm_linf {stm['norm_linf']:.3e}", flush=True)
+    assert stm["norm_linf"] < 1e-4, stm
+
+
+def test_exposure_outside_the_target_grid():
+    from oracle import oracle as O
+    from updc_amd import synth
+    print("case: exposure 2 entirely outside the target grid", flush=True)
+    ds = synth.small_config(n_exp=2, seed=31)
+    c = ds.geoms[1].center_target
+    ds.geoms[1].center_target = (c[0] + 500.0, c[1] + 500.0)
+    ds = oracle_dataset(ds)
+    assert O.parse_linkmap(ds.linkmaps[1])["nlinks"] == 0
+    assert O.parse_linkmap(ds.linkmaps[0])["nlinks"] > 0
+    _check(ds, 1)
+
+
+def test_fully_masked_exposure():
+    from updc_amd import synth
+    print("case: exposure 2 fully masked (all weights 0)", flush=True)
+    ds = oracle_dataset(synth.small_config(n_exp=2, seed=32))
+    ds.wht[1] = np.zeros_like(ds.wht[1])
+    _check(ds, 1)
