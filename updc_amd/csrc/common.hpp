@@ -107,10 +107,42 @@ template <int P, bool INV, class T> __device__ __forceinline__ void bfly_odd(Cx<
 #pragma unroll
     for (int j = 0; j < P; ++j) v[j] = out[j];
 }
+template <bool INV, class T> __device__ __forceinline__ void bfly16(Cx<T> *v) {
+    // 16 = 4 x 4: X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 W4^(n1 k1) x[4 n1 + n2]
+    constexpr double c[10] = {1.0, 0.9238795325112867561282, 0.7071067811865475244008, 0.3826834323650897717285,
+                              0.0, -0.3826834323650897717285, -0.7071067811865475244008, -0.9238795325112867561282,
+                              -1.0, -0.9238795325112867561282};
+    constexpr double s[10] = {0.0, 0.3826834323650897717285, 0.7071067811865475244008, 0.9238795325112867561282,
+                              1.0, 0.9238795325112867561282, 0.7071067811865475244008, 0.3826834323650897717285,
+                              0.0, -0.3826834323650897717285};  // cos, sin (2 pi e / 16), e < 10
+    Cx<T> y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        Cx<T> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        bfly4<INV>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+        for (int k1 = 1; k1 < 4; ++k1) {
+            const int e = n2 * k1;
+            y[n2][k1] = cmul(y[n2][k1], cx<T>((T)c[e], INV ? (T)s[e] : -(T)s[e]));
+        }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        Cx<T> t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        bfly4<INV>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
 template <int R, bool INV, class T> __device__ __forceinline__ void bfly(Cx<T> *v) {
     if constexpr (R == 2) bfly2<INV>(v);
     else if constexpr (R == 4) bfly4<INV>(v);
     else if constexpr (R == 8) bfly8<INV>(v);
+    else if constexpr (R == 16) bfly16<INV>(v);
     else bfly_odd<R, INV>(v);
 }
 
@@ -207,6 +239,9 @@ __device__ __forceinline__ void fft_lds_range(Cx<T> *buf, const FftLen &p, int s
             case 4: fft_stage<4, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
             case 5: fft_stage<5, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
             case 7: fft_stage<7, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
+            case 16:
+                if constexpr (VMAX >= 16) fft_stage<16, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw);
+                break;
             default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, 0, -1, Ns, mg, tw); break;
         }
     }
@@ -281,6 +316,9 @@ __device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg, int
             case 4: fft_stage<4, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
             case 5: fft_stage<5, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
             case 7: fft_stage<7, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
+            case 16:
+                if constexpr (VMAX >= 16) fft_stage<16, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw);
+                break;
             default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
         }
     }

@@ -774,10 +774,16 @@ static constexpr size_t LDS_BUDGET = 150 * 1024;
 // radix goes first: the first stage writes each butterfly's outputs R apart,
 // and for the interleaved row lines an even R (8: 256-byte strides) sends a
 // wave's writes to one LDS bank window, while an odd R spreads them.
-static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullptr, bool odd_first = false) {
+static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullptr, bool odd_first = false,
+                           bool sixteen = false) {
     int n = L, k = 0;
     int rad[UPDC_MAX_STAGES + 8];
     auto push = [&](int r) { if (k < UPDC_MAX_STAGES) rad[k++] = r; };
+    // sixteen: radix 16 where it saves a stage on the power of two (2^4 = 16
+    // against 8 2, 2^7 = 16 8 against 8 8 2); the caller checks it fits
+    int p2 = 0;
+    for (int m = n; m % 2 == 0; m /= 2) ++p2;
+    if (sixteen && !edge && p2 % 3 != 0 && p2 >= 4) { push(16); n /= 16; }
     while (n % 8 == 0) { push(8); n /= 8; }
     if (n % 4 == 0) { push(4); n /= 4; }
     if (n % 2 == 0) { push(2); n /= 2; }
@@ -935,6 +941,13 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (FW < 2 || FH < 1 || (FW & 1)) return "FFT width must be even and >= 2";
     s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1;
     if (!factor_radices(s.M, row, false, nullptr, row_odd_first)) return "FFT width/2 is not 7-smooth";
+    {
+        // a radix-16 stage where it saves one and fits the row tiles' registers
+        FftLen r16{};
+        if (factor_radices(s.M, r16, false, nullptr, row_odd_first, true) && r16.nst < row.nst &&
+            fits(r16, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX))
+            row = r16;
+    }
     if (!factor_radices(FH, col, allow_edge, &col_re)) return "FFT height is not 7-smooth";
     // the edge kernel holds EV / RE butterflies of each thread in registers
     if (col_re && FH / col_re > (ColCfg<T>::EV / col_re) * ColCfg<T>::NT) col_re = 0;
