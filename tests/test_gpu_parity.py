@@ -431,3 +431,32 @@ def test_widest_row_fft_and_the_limit_beyond(small_ds):
     with pytest.raises(UpdcError, match="FFT width too large"):
         c.prepare()
     c.close()
+
+
+def test_radix16_row_plan_matches_oracle(small_ds):
+    """FFT 224 wide: rows of M = 112 = 7 x 16 points take a radix-16 stage
+    (two stages instead of 8 2 7's three) in the persistent FP32 row passes.
+    Every tier against the FP64 oracle at the same shape, within its usual
+    bound; the plain form equals the batched form bit for bit."""
+    from oracle import oracle as O
+    from updc_amd._lib import FLAG_PER_EXPOSURE, check
+    shape = (224, 112)
+    print(f"case: FFT {shape}, radix-16 row stage", flush=True)
+    ref = O.RefSolver(small_ds.W, small_ds.H, oracle_exposures(small_ds), "f64", fft_shape=shape)
+    ref.iterate(3)
+    xs = {}
+    for prec, tol in (("f64", 1e-11), ("mixed", 1e-4), ("f32", 1e-4)):
+        c = _ctx(small_ds, prec, fft=shape)
+        c.prepare()
+        c.iterate(3)
+        xs[prec] = c.estimate(f64=True)
+        st = O.compare_stats(ref.x, xs[prec])
+        print(f"  {prec}: norm_linf {st['norm_linf']:.3e}", flush=True)
+        assert st["norm_linf"] < tol, (prec, st)
+        c.close()
+    p = _ctx(small_ds, "f32", fft=shape)
+    check(p.lib.updc_ctx_set_flags(p.h, p.flags | FLAG_PER_EXPOSURE))
+    p.prepare()
+    p.iterate(3)
+    assert np.array_equal(p.estimate(f64=True), xs["f32"])
+    p.close()
