@@ -70,3 +70,47 @@ def test_fully_masked_exposure():
     ds = oracle_dataset(synth.small_config(n_exp=2, seed=32))
     ds.wht[1] = np.zeros_like(ds.wht[1])
     _check(ds, 1)
+
+
+def test_psf_stored_transposed():
+    """psf_transposed = 1 (updc_core.c:814-819): the stored raster is read as
+    [x][y].  With non-square PSFs (9 high, 7 wide as stored) the FFT shape
+    follows the transposed extents (:1335-1336).  FP64 against the oracle
+    within 1e-11, FreezeMu rows equal; and the closed form: the transposed
+    flag on P is the plain path on the transpose of P."""
+    from oracle import oracle as O
+    from updc_amd import synth
+    from updc_amd.solver import Context, Exposure
+    print("case: PSFs stored transposed (non-square)", flush=True)
+    ds = oracle_dataset(synth.small_config(n_exp=2, seed=41))
+    psfs = [np.ascontiguousarray(p[:, 1:8]) for p in ds.psfs]  # 9 rows x 7 columns as stored
+    ex = oracle_exposures(ds)
+    for k, e in enumerate(ex):
+        e["psf"] = psfs[k]
+    ref = O.RefSolver(ds.W, ds.H, ex, "f64", psf_transposed=1)
+    rows_o, _ = ref.iterate(3)
+
+    def run(precision, transposed, arrays):
+        c = Context(ds.W, ds.H, precision)
+        for k in range(len(ds.geoms)):
+            c.add_exposure(Exposure(ds.sci[k], ds.wht[k], arrays[k], ds.background[k], ds.pedestal[k],
+                                    linkmap=ds.linkmaps[k], psf_transposed=transposed))
+        c.prepare()
+        rows, _ = c.iterate(3)
+        x = c.estimate(f64=True)
+        c.close()
+        return x, rows
+
+    x_t, rows_t = run("f64", True, psfs)
+    st = O.compare_stats(ref.x, x_t)
+    print(f"  f64 transposed vs oracle: norm_linf {st['norm_linf']:.3e}", flush=True)
+    assert st["norm_linf"] < 1e-11, st
+    assert [r[:4] for r in rows_t] == [r[:4] for r in rows_o]
+    x_p, rows_p = run("f64", False, [np.ascontiguousarray(p.T) for p in psfs])
+    sp = O.compare_stats(x_t, x_p)
+    print(f"  f64 transposed flag vs plain on P^T: norm_linf {sp['norm_linf']:.3e}", flush=True)
+    assert sp["norm_linf"] < 1e-12, sp
+    assert [r[:4] for r in rows_p] == [r[:4] for r in rows_t]
+    # the flag matters: the plain path on P itself is a different operator
+    x_n, _ = run("f64", False, psfs)
+    assert O.compare_stats(x_t, x_n)["norm_linf"] > 1e-6
