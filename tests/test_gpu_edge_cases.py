@@ -155,3 +155,59 @@ def test_abs32_linkmaps_iterate_like_delta16(small_ds, prec):
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def _corrupt(img: bytes, what: str) -> bytes:
+    """A DELTA16 ULM2 image with one defect (layout updc_core.c:40-54, 463-483)."""
+    from oracle import oracle as O
+    p = O.parse_linkmap(img)
+    nin, nl = p["nin"], p["nlinks"]
+    b = bytearray(img)
+    pos_off = 64
+    pos_base = pos_off + 4 * (nin + 1)
+    pos_d16 = pos_base + 4 * nin
+    pos_frac = (pos_d16 + 2 * nl + 3) & ~3
+    off = np.frombuffer(b, np.uint32, nin + 1, pos_off)
+    if what == "magic":
+        b[0:4] = b"\0\0\0\0"
+    elif what == "width":
+        np.frombuffer(b, np.uint32, 16, 0)[8] += 1
+    elif what == "truncated":
+        return bytes(b[:-4])
+    elif what == "trailing":
+        return bytes(b) + b"\0"
+    elif what == "last_offset":
+        off[nin] -= 1
+    elif what == "non_monotone":
+        p0 = int(np.nonzero(np.diff(off.astype(np.int64)) > 0)[0][0])  # a pixel with links
+        off[p0 + 1] = off[p0] - 1 if off[p0] > 0 else nl
+        if off[p0 + 1] == nl and p0 + 2 <= nin:
+            off[p0] = nl  # z0 > z1 either way
+    elif what == "target":
+        p0 = int(np.nonzero(np.diff(off.astype(np.int64)) > 0)[0][0])
+        np.frombuffer(b, np.uint32, nin, pos_base)[p0] = p["W"] * p["H"]
+    elif what == "frac":
+        np.frombuffer(b, np.float32, nl, pos_frac)[0] = np.nan
+    return bytes(b)
+
+
+@pytest.mark.parametrize("what,message", [
+    ("magic", "invalid or obsolete LinkMap"), ("width", "invalid or obsolete LinkMap"),
+    ("truncated", "invalid or obsolete LinkMap"), ("trailing", "invalid or obsolete LinkMap"),
+    ("last_offset", "offset/link-count mismatch"), ("non_monotone", "not monotone"),
+    ("target", "outside the target grid"), ("frac", "fraction is not finite")])
+def test_corrupt_linkmap_is_refused(small_ds, what, message):
+    """The reference's LinkMap checks (updc_core.c:485-496, the exact-size rule
+    :463-483, :652) plus the device-side walk of offsets, targets and
+    fractions before any kernel follows them: each defect is refused with a
+    UPDC_FORMAT error naming it."""
+    from updc_amd._lib import UpdcError
+    from updc_amd.solver import Context, Exposure
+    print(f"case: corrupt LinkMap ({what})", flush=True)
+    assert small_ds.linkmaps[0][8] == 2  # DELTA16
+    bad = _corrupt(small_ds.linkmaps[0], what)
+    c = Context(small_ds.W, small_ds.H, "f32")
+    with pytest.raises(UpdcError, match=message):
+        c.add_exposure(Exposure(small_ds.sci[0], small_ds.wht[0], small_ds.psfs[0], small_ds.background[0],
+                                small_ds.pedestal[0], linkmap=bad))
+    c.close()
