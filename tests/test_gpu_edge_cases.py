@@ -211,3 +211,27 @@ def test_corrupt_linkmap_is_refused(small_ds, what, message):
         c.add_exposure(Exposure(small_ds.sci[0], small_ds.wht[0], small_ds.psfs[0], small_ds.background[0],
                                 small_ds.pedestal[0], linkmap=bad))
     c.close()
+
+
+@pytest.mark.parametrize("defect,message", [("negative", "invalid PSF pixel"), ("nan", "invalid PSF pixel"),
+                                            ("zero", "PSF sum is zero/non-finite")])
+def test_invalid_psf_is_refused(small_ds, defect, message):
+    """The reference's PSF checks (updc_core.c:777-789, 809): every raw pixel
+    finite and non-negative, the Neumaier sum positive; refused at set-up."""
+    from updc_amd._lib import UpdcError
+    from updc_amd.solver import Context, Exposure
+    print(f"case: invalid PSF ({defect})", flush=True)
+    psf = small_ds.psfs[1].copy()
+    if defect == "negative":
+        psf[0, 0] = -1e-3
+    elif defect == "nan":
+        psf[2, 3] = np.nan
+    else:
+        psf[:] = 0.0
+    c = Context(small_ds.W, small_ds.H, "mixed")
+    for k in range(len(small_ds.geoms)):
+        c.add_exposure(Exposure(small_ds.sci[k], small_ds.wht[k], psf if k == 1 else small_ds.psfs[k],
+                                small_ds.background[k], small_ds.pedestal[k], linkmap=small_ds.linkmaps[k]))
+    with pytest.raises(UpdcError, match=message):
+        c.prepare()
+    c.close()

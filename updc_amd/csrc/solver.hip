@@ -137,11 +137,13 @@ static int next_smooth(int n) {
 }
 // Neumaier-compensated sums (updc_core.c:777-789): f32 as the reference, f64
 // for the FP64 tier.  Return NaN on an invalid pixel.
+// Neumaier sum of the raw PSF (updc_core.c:777-789); a negative or
+// non-finite pixel fails as the reference does
 template <class T> static T psf_sum(const float *psf, size_t n) {
     T s = 0, c = 0;
     for (size_t i = 0; i < n; ++i) {
         const T v = (T)psf[i];
-        if (!std::isfinite(psf[i]) || psf[i] < 0.0f) return (T)NAN;
+        if (!std::isfinite(psf[i]) || psf[i] < 0.0f) throw Error(UPDC_ERR_NUMERIC, "invalid PSF pixel");
         const T t = s + v;
         if (std::fabs(s) >= std::fabs(v)) c += (s - t) + v;
         else c += (v - t) + s;
