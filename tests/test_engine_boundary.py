@@ -198,3 +198,40 @@ def test_dropin_persistent_caches(small_ds, tmp_path):
     assert "persistent cache hit" in out and "FFT autotune persistent-cache hit" in out
     b = E.run_core(_argv(ds, tmp_path, manifest, 2, 0, prefix="b", autotune=True), env=env, timeout=120)
     assert np.array_equal(a.iterations[2], b.iterations[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("defect,message", [("short", "short float32 input"),
+                                            ("trailing", "float32 input has unexpected trailing bytes")])
+def test_dropin_raw_plane_size_errors(small_ds, tmp_path, defect, message):
+    """Raw float32 planes must hold exactly iw*ih values (updc_core.c:298-315):
+    the label and path go to stderr, then UPDC_ALL: <the reference's message>,
+    exit 1."""
+    ds = small_ds
+    print(f"case: drop-in raw plane {defect}", flush=True)
+    manifest, rows = _files(ds, tmp_path)
+    raw = rows[1].weight_raw.read_bytes()
+    rows[1].weight_raw.write_bytes(raw[:-4] if defect == "short" else raw + b"\0")
+    with pytest.raises(subprocess.CalledProcessError) as ei:
+        E.run_core(_argv(ds, tmp_path, manifest, 1, 0), env=E.solver_env(False), timeout=120)
+    assert ei.value.returncode == 1
+    assert f"UPDC_ALL: {message}" in ei.value.stderr
+    assert f"WHT raw: {rows[1].weight_raw}" in ei.value.stderr
+
+
+@pytest.mark.gpu
+def test_dropin_vertex_map_trailing_bytes_accepted(small_ds, tmp_path):
+    """The reference reads the vertex map with one fread of (ih+1)(iw+1)*2
+    doubles and no trailing-byte check (updc_core.c:536-539): extra bytes
+    after the lattice are ignored, and the solve equals the clean one."""
+    ds = small_ds
+    print("case: drop-in vertex map with trailing bytes", flush=True)
+    manifest, rows = _files(ds, tmp_path)
+    clean = E.run_core(_argv(ds, tmp_path, manifest, 2, 0, prefix="clean"), env=E.solver_env(False), timeout=120)
+    for r in rows:
+        r.vertices.write_bytes(r.vertices.read_bytes() + b"\0" * 24)
+        links = Path(str(r.vertices) + ".links")
+        if links.exists():
+            links.unlink()
+    res = E.run_core(_argv(ds, tmp_path, manifest, 2, 0, prefix="extra"), env=E.solver_env(False), timeout=120)
+    assert np.array_equal(res.iterations[2], clean.iterations[2])

@@ -94,18 +94,25 @@ std::vector<Row> parse_manifest(const char *path, int n) {
     return rows;
 }
 
-// updc_core.c:298-315
+// Raw float32 planes (updc_core.c:298-315: the label and path on stderr,
+// then the failure, exact size) and the f64 vertex map (:536-539: its own
+// messages, no trailing-byte check).
 template <class T> std::vector<T> read_exact(const std::string &path, size_t count, const char *label) {
+    constexpr bool vertex = sizeof(T) == sizeof(double);
+    auto fail = [&](const char *f32msg, const char *vmsg) {
+        if (!vertex) std::fprintf(stderr, "%s: %s\n", label, path.c_str());
+        return DErr(UPDC_ERR_IO, vertex ? vmsg : f32msg);
+    };
     FILE *f = std::fopen(path.c_str(), "rb");
-    if (!f) throw DErr(UPDC_ERR_IO, std::string(label) + ": " + path + ": cannot open input");
+    if (!f) throw fail("cannot open float32 input", "cannot open vertex map");
     std::vector<T> v(count);
     if (std::fread(v.data(), sizeof(T), count, f) != count) {
         std::fclose(f);
-        throw DErr(UPDC_ERR_IO, std::string(label) + ": " + path + ": short input");
+        throw fail("short float32 input", "short vertex map");
     }
-    if (std::fgetc(f) != EOF) {
+    if (!vertex && std::fgetc(f) != EOF) {
         std::fclose(f);
-        throw DErr(UPDC_ERR_IO, std::string(label) + ": " + path + ": input has unexpected trailing bytes");
+        throw fail("float32 input has unexpected trailing bytes", "");
     }
     std::fclose(f);
     return v;
