@@ -235,3 +235,39 @@ def test_dropin_vertex_map_trailing_bytes_accepted(small_ds, tmp_path):
             links.unlink()
     res = E.run_core(_argv(ds, tmp_path, manifest, 2, 0, prefix="extra"), env=E.solver_env(False), timeout=120)
     assert np.array_equal(res.iterations[2], clean.iterations[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("defect,message", [
+    ("fewer", "manifest has fewer rows than expected"),
+    ("more", "manifest has more rows than expected"),
+    ("fields", "manifest requires science, weight, psf, psf_width, psf_height, vertices, background, pedestal"),
+    ("psf_width", "invalid PSF dimensions in manifest"),
+    ("pedestal", "invalid background/pedestal in manifest")])
+def test_dropin_manifest_errors(small_ds, tmp_path, defect, message):
+    """The manifest rules of updc_core.c:426-461: exactly N rows (any further
+    line, even a blank one, is an extra row), eight tab-separated fields, PSF
+    sizes >= 1, finite background, finite non-negative pedestal."""
+    ds = small_ds
+    print(f"case: drop-in manifest {defect}", flush=True)
+    manifest, _ = _files(ds, tmp_path)
+    lines = manifest.read_text().splitlines()
+    if defect == "fewer":
+        lines = lines[:-1]
+    elif defect == "more":
+        lines = lines + [""]
+    elif defect == "fields":
+        lines[0] = "\t".join(lines[0].split("\t")[:7])
+    elif defect == "psf_width":
+        f = lines[0].split("\t")
+        f[3] = "0"
+        lines[0] = "\t".join(f)
+    else:
+        f = lines[1].split("\t")
+        f[7] = "-1"
+        lines[1] = "\t".join(f)
+    manifest.write_text("\n".join(lines) + "\n")
+    with pytest.raises(subprocess.CalledProcessError) as ei:
+        E.run_core(_argv(ds, tmp_path, manifest, 1, 0), env=E.solver_env(False), timeout=120)
+    assert ei.value.returncode == 1
+    assert f"UPDC_ALL: {message}" in ei.value.stderr
