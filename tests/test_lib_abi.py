@@ -4,6 +4,8 @@ import ctypes
 import re
 from pathlib import Path
 
+import pytest
+
 ROOT = Path(__file__).resolve().parents[1]
 
 
@@ -50,3 +52,26 @@ def test_dropin_psf_projector_self_test():
     assert r.returncode == 0 and "UPDC PSF projector self-test: OK" in r.stdout
     bad = subprocess.run([str(_lib.PROJECTOR_PATH), "a"], capture_output=True, text=True)
     assert bad.returncode == 1 and bad.stderr.startswith("UPDC_PSF_PROJECT: usage")
+
+
+_GOOD_ARGV = ["m.txt", "96", "48", "48", "48", "2", "0", "2", "it", "w.bin", "1", "1", "1", "0", "0", "0", "wis", "1"]
+
+
+@pytest.mark.parametrize("pos,value", [(1, "0"), (2, "0"), (3, "0"), (4, "0"), (5, "0"), (7, "0"), (10, "0"),
+                                       (11, "0"), (12, "0"), (17, "-1"), (1, "abc")])
+def test_dropin_argument_ranges(pos, value):
+    """The reference's range check on argv (updc_core.c:1294-1296) runs before
+    any file or device is touched: each out-of-range value exits 1 with its
+    message, from the executable and from updc_all alike."""
+    import subprocess
+    from updc_amd import _lib
+    argv = list(_GOOD_ARGV)
+    argv[pos] = value
+    r = subprocess.run([str(_lib.EXE_PATH)] + argv, capture_output=True, text=True)
+    assert r.returncode == 1
+    assert r.stderr.strip().endswith("UPDC_ALL: invalid dimensions/exposures/iterations/workers/save_every")
+    lib = _lib.load()
+    args = [a.encode() if i in (0, 8, 9, 16) else int(a) if a.lstrip("-").isdigit() else 0
+            for i, a in enumerate(argv)]
+    assert lib.updc_all(*args) != 0
+    assert lib.updc_last_error().decode() == "invalid dimensions/exposures/iterations/workers/save_every"
