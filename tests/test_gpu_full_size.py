@@ -6,7 +6,8 @@ this test rests on properties that hold at any size:
   max |x|) of FP64 after 40 iterations;
 - batched FP64 and the plain per-exposure FP64 form (the reference's dataflow,
   updc_core.c:1410-1458) give the same estimate and FreezeMu rows bit for bit;
-- the fused and split adjoint forms give the same estimate bit for bit;
+- the fused and split adjoint forms, and hipGraph replay and eager launches,
+  give the same estimate bit for bit;
 - FreezeMu rows number iterations x exposures, in (iteration, exposure) order
   (updc_core.c:1425-1431), and a frozen pixel stays frozen: each row's total
   is the previous total plus the newly frozen (updc_core.c:1162-1164);
@@ -29,10 +30,11 @@ def full_ds():
     return make_dataset(2)
 
 
-def _run(ds, precision, plain=False, fused=False):
+def _run(ds, precision, plain=False, fused=False, graph=True):
     from bench import solve_ctx
-    print(f"  solve {precision} plain={plain} fused={fused}", flush=True)
-    c = solve_ctx(ds, precision, graph=not plain, plain=plain)
+    graph = graph and not plain
+    print(f"  solve {precision} plain={plain} fused={fused} graph={graph}", flush=True)
+    c = solve_ctx(ds, precision, graph=graph, plain=plain)
     if fused:
         c.set_split_adjoint(False)
     x0, _ = c.initial()
@@ -55,6 +57,9 @@ def test_full_size_tiers_and_forms(full_ds):
     xmf, rowsmf = _run(full_ds, "mixed", fused=True)
     assert np.array_equal(xm, xmf)
     assert rowsm == rowsmf
+    xme, rowsme = _run(full_ds, "mixed", graph=False)
+    assert np.array_equal(xm, xme)
+    assert rowsm == rowsme
     st = O.compare_stats(x64, xm)
     print(f"  mixed vs f64 after {ITERS}: norm_linf {st['norm_linf']:.3e} rms {st['rms']:.3e}", flush=True)
     assert st["norm_linf"] <= 1e-5, st
