@@ -382,16 +382,10 @@ class RefSolver:
             self.coverage = cov.reshape(self.H, self.W)
         # denominator (updc_core.c:1026-1086)
         if f32:
-            acc = np.zeros((self.FH, self.FW // 2 + 1), np.complex64)
-            for e in self.ex:
-                sc = sensitivity_scatter(e["linkmap"], self.nout, e["w"]).reshape(self.H, self.W)
-                accumulate_adjoint(acc, self._rfft(sc), e["P"])
-            self.D = self._irfft_crop(acc)
+            self.D = self.blur_adjoint([sensitivity_scatter(e["linkmap"], self.nout, e["w"]).reshape(self.H, self.W)
+                                        for e in self.ex])
         else:
-            acc = np.zeros((self.FH, self.FW // 2 + 1), np.complex128)
-            for e in self.ex:
-                acc += self._rfft(self._scatter64(e, e["w"])) * np.conj(e["P"])
-            self.D = self._irfft_crop(acc)
+            self.D = self.blur_adjoint([self._scatter64(e, e["w"]) for e in self.ex])
         self.max_sens = float(self.D.max())
         if not (self.max_sens > 0):
             raise FloatingPointError("invalid H^T W sensitivity")
@@ -407,6 +401,26 @@ class RefSolver:
         contrib = lm["frac"].astype(np.float64) * blur.ravel()[lm["target"]]
         return np.bincount(lm["pixel"], contrib, e["s"].size)
 
+    def _new_acc(self) -> np.ndarray:
+        return np.zeros((self.FH, self.FW // 2 + 1), np.complex64 if self.prec == "f32" else np.complex128)
+
+    def _accumulate(self, acc: np.ndarray, img: np.ndarray, e: dict) -> None:
+        """acc += R2C(pad img) . conj(P^_k): accumulate_adjoint_spectrum
+        (updc_core.c:985-995), f32 without FMA in the C restatement."""
+        if self.prec == "f32":
+            accumulate_adjoint(acc, self._rfft(img), e["P"])
+        else:
+            acc += self._rfft(img) * np.conj(e["P"])
+
+    def blur_adjoint(self, imgs) -> np.ndarray:
+        """sum_k crop(C2R(R2C(pad img_k) . conj(P^_k))) / nf: the adjoint half
+        of the iteration (updc_core.c:985-995, 1280-1287) and of the
+        denominator (updc_core.c:1026-1086), the adjoint of blur()."""
+        acc = self._new_acc()
+        for e, img in zip(self.ex, imgs):
+            self._accumulate(acc, np.asarray(img).reshape(self.H, self.W), e)
+        return self._irfft_crop(acc)
+
     def blur(self, x=None) -> list[np.ndarray]:
         X = self._rfft(self.x if x is None else x)
         out = []
@@ -420,14 +434,14 @@ class RefSolver:
         for it in range(n):
             X = self._rfft(self.x)
             f32 = self.prec == "f32"
-            acc = np.zeros((self.FH, self.FW // 2 + 1), np.complex64 if f32 else np.complex128)
+            acc = self._new_acc()
             for k, e in enumerate(self.ex):
                 prod = spectrum_multiply(X, e["P"]) if f32 else X * e["P"]
                 bl = self._irfft_crop(prod)
                 if f32:
                     wr, sc, (nn, nt, mn) = fused_detector(e["linkmap"], self.nout, e["s"], e["w"], e["frozen"], bl,
                                                           float(e["mo_f"]), float(e["ped_f"]))
-                    accumulate_adjoint(acc, self._rfft(sc.reshape(self.H, self.W)), e["P"])
+                    self._accumulate(acc, sc.reshape(self.H, self.W), e)
                 else:
                     lm = e["lm"]
                     act = (e["w"] > 0) & (np.diff(lm["offset"].astype(np.int64)) > 0)
@@ -444,7 +458,7 @@ class RefSolver:
                     wr = np.where(act, e["w"] * ratio, 0.0)
                     nn, nt = int(newly.sum()), int(fz.sum())
                     mn = float(raw[act].min()) if act.any() else float("inf")
-                    acc += self._rfft(self._scatter64(e, wr)) * np.conj(e["P"])
+                    self._accumulate(acc, self._scatter64(e, wr), e)
                 rows.append((it + 1, k + 1, nn, nt, mn))
             Cc = self._irfft_crop(acc)
             if f32:

@@ -22,6 +22,11 @@
  *       cases, the adjoint identity <S u, v> = <u, S^T v>, FreezeMu invariants.
  *   orc_update / orc_drizzle / orc_weight_scale / orc_psf_* -> closed forms
  *       on hand-made inputs (tests/test_oracle_stages.py).
+ *   orc_spectrum_multiply / orc_accumulate_adjoint -> exact complex products
+ *       of the same float32 inputs within the rounding bound, with a
+ *       conjugation error shown to fall far outside it; the adjoint identity
+ *       of the whole blur / adjoint pair built on them
+ *       (tests/test_oracle_adjoint.py).
  */
 #include <math.h>
 #include <stdint.h>
