@@ -193,8 +193,24 @@ int updc_get_linkmap(updc_ctx *ctx, int k, void *out, size_t capacity);
  * drizzle, denominator and its maximum (updc_core.c:1367-1398). */
 int updc_prepare(updc_ctx *ctx);
 /* Run n iterations.  rows (optional) receives n*N freeze records, info
- * (optional) n per-iteration summaries. */
+ * (optional) n per-iteration summaries.  When a numeric guard of the
+ * reference fires (UPDC_ERR_NUMERIC: non-finite forward mean or ratio,
+ * updc_core.c:1160-1169; non-finite or substantially negative correction,
+ * non-finite estimate, :1441-1453), the iterations after the failing one do
+ * not run, the message names the first failure the reference's serial loops
+ * meet (kind, iteration, exposure and pixel or target), and rows/info hold
+ * only what the reference would have written before stopping: info for the
+ * completed iterations, freeze rows for those plus, for a failure in the
+ * update, the failing iteration's rows (the reference writes them before its
+ * update).  Entries after that are zero; updc_get_iterate_progress gives the
+ * counts.  After a detector-stage failure the estimate is the last completed
+ * iteration's; after an update failure it holds the failing update. */
 int updc_iterate(updc_ctx *ctx, int n, updc_freeze_row *rows, updc_iter_info *info);
+/* Of the last updc_iterate call: completed = iterations whose update ran,
+ * freeze_rows = iterations whose freeze rows are valid (the UPDC_ALL loop's
+ * per-iteration CSV and stdout lines, updc_core.c:1410-1479).  Both equal n
+ * after a successful call.  Does not reset updc_last_error(). */
+int updc_get_iterate_progress(updc_ctx *ctx, int *completed, int *freeze_rows);
 
 /* Results and setup products (host copies; f32 outputs are what the reference
  * writes, f64 variants return the working precision). */
@@ -230,6 +246,11 @@ int updc_stage_detector(updc_ctx *ctx, int k, const double *blur, double *wr_out
 int updc_stage_adjoint(updc_ctx *ctx, const double *v, double *out);
 /* S_k^T v for one exposure (H*W), via the transposed gather. */
 int updc_stage_scatter(updc_ctx *ctx, int k, const double *v, double *out);
+/* S_k^T v_k for every exposure at once (out: N*H*W) through the iteration's
+ * own adjoint-image launch: the scatter half of the detector loop,
+ * updc_core.c:1171-1176 (v is the flattened per-pixel vector, sum of nin;
+ * pixels with w <= 0 contribute nothing). */
+int updc_stage_adjoint_images(updc_ctx *ctx, const double *v, double *out);
 /* Zero every FreezeMu mask. */
 int updc_reset_frozen(updc_ctx *ctx);
 

@@ -37,3 +37,17 @@ def oracle_exposures(ds):
 def small_ds():
     from updc_amd import synth
     return oracle_dataset(synth.small_config())
+
+
+def as_abs32(img: bytes) -> bytes:
+    """The same links and fractions in the ABS32 encoding (ULM2 encoding 1,
+    updc_core.c:40-54, 463-483): header, offset[nin+1], target[L], frac[L]."""
+    from oracle import oracle as O
+    p = O.parse_linkmap(img)
+    hdr = np.frombuffer(img[:64], np.uint8).copy()
+    hdr.view(np.uint32)[2] = 1  # encoding
+    parts = [hdr.tobytes(), p["offset"].astype("<u4").tobytes(), p["target"].astype("<u4").tobytes(),
+             p["frac"].astype("<f4").tobytes()]
+    out = b"".join(parts)
+    assert len(out) == 64 + 4 * (p["nin"] + 1) + 8 * p["nlinks"]
+    return out

@@ -15,7 +15,7 @@ rows all zero.
 import numpy as np
 import pytest
 
-from conftest import oracle_dataset, oracle_exposures
+from conftest import as_abs32, oracle_dataset, oracle_exposures
 
 pytestmark = pytest.mark.gpu
 
@@ -116,20 +116,6 @@ def test_psf_stored_transposed():
     assert O.compare_stats(x_t, x_n)["norm_linf"] > 1e-6
 
 
-def _as_abs32(img: bytes) -> bytes:
-    """The same links and fractions in the ABS32 encoding (ULM2 encoding 1,
-    updc_core.c:40-54, 463-483): header, offset[nin+1], target[L], frac[L]."""
-    from oracle import oracle as O
-    p = O.parse_linkmap(img)
-    hdr = np.frombuffer(img[:64], np.uint8).copy()
-    hdr.view(np.uint32)[2] = 1  # encoding
-    parts = [hdr.tobytes(), p["offset"].astype("<u4").tobytes(), p["target"].astype("<u4").tobytes(),
-             p["frac"].astype("<f4").tobytes()]
-    out = b"".join(parts)
-    assert len(out) == 64 + 4 * (p["nin"] + 1) + 8 * p["nlinks"]
-    return out
-
-
 @pytest.mark.parametrize("prec", ["f32", "mixed"])
 def test_abs32_linkmaps_iterate_like_delta16(small_ds, prec):
     """Geometry picks DELTA16 below a 65536-wide target, so ABS32 LinkMaps
@@ -143,7 +129,7 @@ def test_abs32_linkmaps_iterate_like_delta16(small_ds, prec):
     for enc in ("delta16", "abs32"):
         c = Context(small_ds.W, small_ds.H, prec)
         for k in range(len(small_ds.geoms)):
-            lm = small_ds.linkmaps[k] if enc == "delta16" else _as_abs32(small_ds.linkmaps[k])
+            lm = small_ds.linkmaps[k] if enc == "delta16" else as_abs32(small_ds.linkmaps[k])
             if enc == "abs32":
                 assert O.parse_linkmap(lm)["encoding"] == 1
             c.add_exposure(Exposure(small_ds.sci[k], small_ds.wht[k], small_ds.psfs[k], small_ds.background[k],

@@ -76,6 +76,7 @@ SIGNATURES = {
     "updc_get_linkmap": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "updc_prepare": (C.c_int, [_P]),
     "updc_iterate": (C.c_int, [_P, C.c_int, C.POINTER(FreezeRow), C.POINTER(IterInfo)]),
+    "updc_get_iterate_progress": (C.c_int, [_P, _I, _I]),
     "updc_get_estimate": (C.c_int, [_P, _F]),
     "updc_get_estimate_f64": (C.c_int, [_P, _D]),
     "updc_set_estimate_f64": (C.c_int, [_P, _D]),
@@ -92,6 +93,7 @@ SIGNATURES = {
     "updc_stage_detector": (C.c_int, [_P, C.c_int, _D, _D, _D, _D]),
     "updc_stage_adjoint": (C.c_int, [_P, _D, _D]),
     "updc_stage_scatter": (C.c_int, [_P, C.c_int, _D, _D]),
+    "updc_stage_adjoint_images": (C.c_int, [_P, _D, _D]),
     "updc_reset_frozen": (C.c_int, [_P]),
     "updc_fft_r2c_f64": (C.c_int, [_P, _D, _D]),
     "updc_fft_c2r_f64": (C.c_int, [_P, _D, _D]),

@@ -274,15 +274,18 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
             const double tc = now();
             const int rc = updc_iterate(g.c, chunk, fr.data(), info.data());
             const double el = now() - tc;
-            for (int i = 0; i < chunk; ++i) {
+            // after a numeric failure only the iterations the reference would
+            // have finished print and log (updc_core.c:1425-1431, 1473-1478)
+            int completed = chunk, with_rows = chunk;
+            if (rc != UPDC_OK) check(updc_get_iterate_progress(g.c, &completed, &with_rows));
+            for (int i = 0; i < with_rows; ++i) {
                 for (int k = 0; k < n; ++k) {
                     const updc_freeze_row &r = fr[(size_t)i * n + k];
-                    if (rc != UPDC_OK && r.n_new_frozen == 0 && r.n_total_frozen == 0 && std::isinf(r.min_raw_mu))
-                        continue;
                     std::fprintf(fcsv, "%d,%d,%llu,%llu,%.9g,%.9g,1\n", done + i + 1, k + 1,
                                  (unsigned long long)r.n_new_frozen, (unsigned long long)r.n_total_frozen,
                                  (double)(float)r.min_raw_mu, (double)1.0e-12f);
                 }
+                if (i >= completed) break;
                 iter_s[done + i] = el / chunk;
                 if (!quiet)
                     std::printf("UPDC iteration %d/%d: signed sum %.9g, negative %llu/%zu, frozen +%llu total %llu, "

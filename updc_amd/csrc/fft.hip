@@ -322,23 +322,34 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
     const T thr = sizeof(T) == 4 ? (T)1e-8f : (T)1e-8;
     const T neg = sizeof(T) == 4 ? (T)(-1.0e-6f) : (T)(-1.0e-6);
     double sum = 0.0;
-    unsigned long long nneg = 0, badidx = ~0ull;
+    unsigned long long nneg = 0, badkey = ~0ull;
     unsigned int bad = 0;
+    // the detector stage of this iteration, or an earlier iteration, failed:
+    // the reference never reaches this update, so x stays as it is
+    const bool hold = idiag->halt || (idiag->err & (UPDC_E_MU | UPDC_E_RATIO));
     auto upd = [&](size_t t, T c) __attribute__((always_inline)) {
+        if (hold) return;
         const T d = D[t];
         TF xv = x[t];
+        unsigned int b = 0;
         if (d > thr) {
             T fct = c / d;
-            if (!isfinite(fct)) { bad |= UPDC_E_FACTOR; badidx = badidx < t ? badidx : t; }
+            if (!isfinite(fct)) b = b ? b : UPDC_E_FACTOR;
             if (fct < (T)0) {
-                if (c < neg * max_sens) { bad |= UPDC_E_NEGCORR; badidx = badidx < t ? badidx : t; }
+                if (c < neg * max_sens) b = b ? b : UPDC_E_NEGCORR;
                 fct = (T)0;
             }
             xv = (TF)((T)xv * fct);
         } else {
             xv = (TF)0;
         }
-        if (!isfinite(xv)) { bad |= UPDC_E_ESTIMATE; badidx = badidx < t ? badidx : t; }
+        if (!isfinite(xv)) b = b ? b : UPDC_E_ESTIMATE;
+        if (b) {
+            bad |= b;
+            const unsigned long long key =
+                err_key_update(t, b == UPDC_E_FACTOR ? 0u : b == UPDC_E_NEGCORR ? 1u : 2u);
+            badkey = badkey < key ? badkey : key;
+        }
         x[t] = xv;
         sum += (double)xv;
         nneg += xv < (TF)0 ? 1 : 0;
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
 
     if (bad) {
         atomicOr(&idiag->err, bad);
-        atomicMin(&idiag->err_index, badidx);
+        atomicMin(&idiag->err_key, badkey);
     }
     // workgroup reduction in LDS (the transform is done with z)
     __syncthreads();

@@ -40,14 +40,31 @@ struct ExpDiag {
 // Per-iteration diagnostics of the update kernel (updc_core.c:1438-1456) and
 // error word.  err bits: 1 non-finite forward mean, 2 non-finite ratio,
 // 4 non-finite correction factor, 8 substantially negative correction,
-// 16 non-finite estimate.  err_index keeps the smallest failing index.
+// 16 non-finite estimate.  err_key is the smallest failure key (atomicMin):
+// the failure the reference's serial loops meet first, with the kind the
+// reference checks first at that element.
+//   detector stage: (exposure << 34) | (pixel << 2) | rank, rank 0 non-finite
+//                   forward mean, 1 non-finite ratio (updc_core.c:1160-1169);
+//   update:         (target << 2) | rank, rank 0 non-finite factor, 1
+//                   substantially negative, 2 non-finite estimate
+//                   (updc_core.c:1441-1453).
+// halt is sticky within one updc_iterate call: the bookkeeping kernel sets it
+// once an iteration has failed, and the detector and update kernels of later
+// iterations then change nothing (no freeze, no estimate write), so the state
+// stops at the failing iteration, as the reference stops there.
 struct IterDiag {
     double signed_sum;
     unsigned long long n_negative;
     unsigned int err;
-    unsigned int pad;
-    unsigned long long err_index;
+    unsigned int halt;
+    unsigned long long err_key;
 };
+__host__ __device__ __forceinline__ unsigned long long err_key_detector(unsigned k, unsigned p, unsigned rank) {
+    return ((unsigned long long)k << 34) | ((unsigned long long)p << 2) | rank;
+}
+__host__ __device__ __forceinline__ unsigned long long err_key_update(unsigned long long t, unsigned rank) {
+    return (t << 2) | rank;
+}
 #define UPDC_E_MU 1u
 #define UPDC_E_RATIO 2u
 #define UPDC_E_FACTOR 4u

@@ -135,10 +135,9 @@ static int next_smooth(int n) {
     for (;; ++n)
         if (seven_smooth(n)) return n;
 }
-// Neumaier-compensated sums (updc_core.c:777-789): f32 as the reference, f64
-// for the FP64 tier.  Return NaN on an invalid pixel.
-// Neumaier sum of the raw PSF (updc_core.c:777-789); a negative or
-// non-finite pixel fails as the reference does
+// Neumaier-compensated sum of the raw PSF (updc_core.c:777-789): f32 as the
+// reference, f64 for the FP64 tier.  A negative or non-finite pixel fails as
+// the reference does
 template <class T> static T psf_sum(const float *psf, size_t n) {
     T s = 0, c = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -224,11 +223,16 @@ struct PipeBase {
     virtual void stage_detector(Ctx &c, int k, const double *blur, double *wr, double *scatter, double *stats) = 0;
     virtual void stage_adjoint(Ctx &c, const double *v, double *out) = 0;
     virtual void stage_scatter(Ctx &c, int k, const double *v, double *out) = 0;
+    virtual void stage_adjoint_images(Ctx &c, const double *v, double *out) = 0;
     virtual void fft_r2c(Ctx &c, const double *img, double *spec) = 0;
     virtual void fft_c2r(Ctx &c, const double *spec, double *img) = 0;
     virtual void get_frozen(Ctx &c, int k, uint8_t *out) = 0;
     virtual void reset_frozen(Ctx &c) = 0;
     virtual void invalidate_graph() = 0;  // after a flag change that alters the launches
+    // The plain form's extra spectra, allocated here (at prepare or when the
+    // flag is switched on), never inside launch_iteration_plain, which may
+    // run under stream capture where hipMalloc is not allowed.
+    virtual void ensure_plain_buffers(Ctx &c) = 0;
     // device-pointer operators (element type TF), on the caller's stream
     virtual void op_r2c(Ctx &c, const void *img, int rows, int cols, int pitch, void *spec, hipStream_t st) = 0;
     virtual void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double scale,
@@ -283,6 +287,8 @@ struct Ctx {
     std::vector<std::unique_ptr<HostExp>> ex;
     hipStream_t st = nullptr;
     bool prepared = false;
+    int iter_done = 0;  // the last updc_iterate: iterations completed (estimate updated)
+    int iter_rows = 0;  // ... and iterations whose freeze rows are valid
     float weight_scale = 0;
     unsigned long long masked = 0;
     std::unique_ptr<PipeBase> pipe;
@@ -666,6 +672,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         ticket.alloc(1);
         ctr.alloc(1);
         HIPCHK(fill<unsigned int>(ticket.p, 0u, 1, c.st));
+        if (c.flags & UPDC_FLAG_PER_EXPOSURE) ensure_plain_buffers(c);
         HIPCHK(hipStreamSynchronize(c.st));
     }
 
@@ -680,10 +687,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const size_t kstride = fft.spec_elems(c.H), fstride = fft.spec_elems(s.FH);
         const TF sc = scale(c);
         auto mark = [&](int i) { if (ev) HIPCHK(hipEventRecord(ev[i], c.st)); };
-        if (!spec_tmp.p) {
-            spec_tmp.alloc(fstride);
-            if (!spec_acc.p) spec_acc.alloc(fstride);
-        }
+        if (!spec_tmp.p || !spec_acc.p) throw Error(UPDC_ERR_STATE, "per-exposure form buffers not allocated");
         Cx<TF> *xh = spec_x.p;  // full FH-row spectrum (pitch rp(FH)) after the column pass
         mark(0);
         HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, xh, 0, s.FH, c.H, 1, c.st));
@@ -768,6 +772,12 @@ template <class TF, class TA> struct Pipe : PipeBase {
     void invalidate_graph() override {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
+    }
+    void ensure_plain_buffers(Ctx &c) override {
+        const size_t fstride = fft.spec_elems(fft.s.FH);
+        if (!spec_tmp.p) spec_tmp.alloc(fstride);
+        if (!spec_acc.p) spec_acc.alloc(fstride);
+        (void)c;
     }
 
     void iterate(Ctx &c, int niter, updc_freeze_row *rows, updc_iter_info *info) override {
@@ -858,7 +868,20 @@ template <class TF, class TA> struct Pipe : PipeBase {
         std::vector<IterDiag> hi(niter);
         HIPCHK(hipMemcpy(hd.data(), diag_hist.p, hd.size() * sizeof(ExpDiag), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(hi.data(), idiag_hist.p, hi.size() * sizeof(IterDiag), hipMemcpyDeviceToHost));
-        for (int it = 0; it < niter; ++it) {
+        // The first failing iteration f (0-based), if any.  The reference stops
+        // there: a detector-stage failure before writing f's freeze rows
+        // (updc_core.c:1160-1169, rows at :1425-1431), an update failure after
+        // them (:1441-1453).  Later iterations did not run on the device either
+        // (the halt flag).
+        int fail_it = niter;
+        for (int it = 0; it < niter; ++it)
+            if (hi[it].err) { fail_it = it; break; }
+        const bool det_fail = fail_it < niter && (hi[fail_it].err & (UPDC_E_MU | UPDC_E_RATIO));
+        c.iter_done = fail_it;
+        c.iter_rows = fail_it < niter && !det_fail ? fail_it + 1 : fail_it;
+        if (rows) std::memset((void *)rows, 0, (size_t)niter * n * sizeof(updc_freeze_row));
+        if (info) std::memset((void *)info, 0, (size_t)niter * sizeof(updc_iter_info));
+        for (int it = 0; it < c.iter_rows; ++it) {
             uint64_t nn = 0, nt = 0;
             double mn = INFINITY;
             for (int k = 0; k < n; ++k) {
@@ -876,7 +899,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
                 nt += d.n_total;
                 mn = std::min(mn, m);
             }
-            if (info) {
+            if (info && it < c.iter_done) {
                 info[it].iteration = it + 1;
                 info[it].signed_sum = hi[it].signed_sum;
                 info[it].n_negative = hi[it].n_negative;
@@ -885,18 +908,24 @@ template <class TF, class TA> struct Pipe : PipeBase {
                 info[it].min_raw_mu = mn;
                 info[it].ms_total = total_ms / std::max(1, niter);
             }
-            const unsigned int eb = hi[it].err;
-            if (eb) {
-                char msg[256];
-                const char *what = (eb & UPDC_E_MU) ? "non-finite forward mean"
-                                   : (eb & UPDC_E_RATIO) ? "non-finite y/raw_mu ratio"
-                                   : (eb & UPDC_E_FACTOR) ? "non-finite multiplicative correction"
-                                   : (eb & UPDC_E_NEGCORR) ? "substantially negative multiplicative correction"
-                                                           : "non-finite signed UPDC estimate";
-                std::snprintf(msg, sizeof msg, "%s (iteration %d, index %llu)", what, it + 1,
-                              (unsigned long long)hi[it].err_index);
-                throw Error(UPDC_ERR_NUMERIC, msg);
+        }
+        if (fail_it < niter) {
+            // the kind and place the reference's serial loops meet first
+            const unsigned long long key = hi[fail_it].err_key;
+            const unsigned rank = (unsigned)(key & 3u);
+            char msg[256];
+            if (det_fail) {
+                std::snprintf(msg, sizeof msg, "%s (iteration %d, exposure %u, pixel %llu)",
+                              rank == 0 ? "non-finite forward mean" : "non-finite y/raw_mu ratio", fail_it + 1,
+                              (unsigned)(key >> 34) + 1u, (unsigned long long)((key >> 2) & 0xffffffffull));
+            } else {
+                std::snprintf(msg, sizeof msg, "%s (iteration %d, target %llu)",
+                              rank == 0   ? "non-finite multiplicative correction"
+                              : rank == 1 ? "substantially negative multiplicative correction"
+                                          : "non-finite signed UPDC estimate",
+                              fail_it + 1, (unsigned long long)(key >> 2));
             }
+            throw Error(UPDC_ERR_NUMERIC, msg);
         }
     }
 
@@ -1029,6 +1058,15 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(csc_apply<TF>(d_ex.p + k, wr.p, c.nout(), o.p, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(o.p, c.nout(), out, c);
+    }
+    // The iteration's own adjoint-image launch (csc_image over every exposure,
+    // as in launch_iteration), written into the blur images it uses there.
+    void stage_adjoint_images(Ctx &c, const double *v, double *out) override {
+        const size_t nout = c.nout();
+        from_host_f64(v, c.npix(), wr.p, c);
+        HIPCHK(csc_image<TF>(d_ex.p, N(c), wr.p, nout, blur.p, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        to_host_f64(blur.p, (size_t)N(c) * nout, out, c);
     }
     void fft_r2c(Ctx &c, const double *img, double *spec) override {
         const FftShape &s = fft.s;
@@ -1503,6 +1541,10 @@ int updc_ctx_set_flags(updc_ctx *ctx, unsigned flags) {
         need(ctx != nullptr, UPDC_ERR_ARG, "null context");
         const unsigned shape = UPDC_FLAG_PER_EXPOSURE | UPDC_FLAG_FUSED_ADJOINT;
         if (((ctx->flags ^ flags) & shape) && ctx->pipe) ctx->pipe->invalidate_graph();
+        if ((flags & UPDC_FLAG_PER_EXPOSURE) && ctx->prepared) {
+            HIPCHK(hipSetDevice(ctx->device));
+            ctx->pipe->ensure_plain_buffers(*ctx);
+        }
         ctx->flags = flags;
     });
 }
@@ -1788,9 +1830,17 @@ int updc_iterate(updc_ctx *ctx, int n, updc_freeze_row *rows, updc_iter_info *in
     return guarded([&] {
         NEED_PREPARED(ctx);
         need(n >= 1, UPDC_ERR_ARG, "invalid iteration count");
+        ctx->iter_done = ctx->iter_rows = 0;
         HIPCHK(hipSetDevice(ctx->device));
         ctx->pipe->iterate(*ctx, n, rows, info);
     });
+}
+// Leaves updc_last_error() as it is: callers read it after a failed iterate.
+int updc_get_iterate_progress(updc_ctx *ctx, int *completed, int *freeze_rows) {
+    if (!ctx) return UPDC_ERR_ARG;
+    if (completed) *completed = ctx->iter_done;
+    if (freeze_rows) *freeze_rows = ctx->iter_rows;
+    return UPDC_OK;
 }
 int updc_get_estimate(updc_ctx *ctx, float *out) {
     return guarded([&] { NEED_PREPARED(ctx); need(out, UPDC_ERR_ARG, "null"); ctx->pipe->get_estimate(*ctx, nullptr, out); });
@@ -1871,6 +1921,15 @@ int updc_stage_scatter(updc_ctx *ctx, int k, const double *v, double *out) {
         ctx->pipe->stage_scatter(*ctx, k, v, out);
     });
 }
+int updc_stage_adjoint_images(updc_ctx *ctx, const double *v, double *out) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(v && out, UPDC_ERR_ARG, "null");
+        HIPCHK(hipSetDevice(ctx->device));
+        ctx->pipe->stage_adjoint_images(*ctx, v, out);
+    });
+}
+
 // ---- device-pointer operators
 int updc_op_layout(updc_ctx *ctx, int *fft_width, int *fft_height, int *spectrum_columns, int *spectrum_pitch,
                    int *element_bytes) {

@@ -577,6 +577,8 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
     __shared__ uint4 s_fv[GATHER_FV];
     __shared__ unsigned int r_new[GATHER_NT / 64], r_tot[GATHER_NT / 64];
     __shared__ TA r_min[GATHER_NT / 64];
+    // an earlier iteration of this call failed: change nothing (uniform exit)
+    if (idiag->halt) return;
     const int k = kbeg + blockIdx.y;
     const ExpDev &e = ex[k];
     const uint32_t nin = e.nin;
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
     const uint32_t *s32 = (const uint32_t *)s_lv;
     const float *sfr = (const float *)s_fv;
     unsigned int nnew = 0, ntot = 0, bad = 0;
-    unsigned long long badidx = ~0ull;
+    unsigned long long badkey = ~0ull;
     TA mn = (TA)INFINITY;
     const uint32_t step = gridDim.x * GATHER_NT;
     uint32_t p0 = blockIdx.x * GATHER_NT;
@@ -704,7 +706,8 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
                 unsigned int b = 0;
                 if (!isfinite(raw)) b |= UPDC_E_MU;
                 mn = raw < mn ? raw : mn;
-                if (!fz && !(raw > (TA)0)) {
+                // the reference fails before its freeze test on a non-finite mean
+                if (!fz && !b && !(raw > (TA)0)) {
                     fz = 1;
                     frozen[gp] = 1;
                     ++nnew;
@@ -723,7 +726,8 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
                 out = (TF)((TA)wp * ratio);
                 if (b) {
                     bad |= b;
-                    badidx = badidx < p ? badidx : p;
+                    const unsigned long long key = err_key_detector((unsigned)k, p, (b & UPDC_E_MU) ? 0u : 1u);
+                    badkey = badkey < key ? badkey : key;
                 }
             }
             wr[gp] = out;
@@ -757,7 +761,7 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
     }
     if (bad) {
         atomicOr(&idiag->err, bad);
-        atomicMin(&idiag->err_index, badidx);
+        atomicMin(&idiag->err_key, badkey);
     }
 }
 template <class TF, class TA>
@@ -814,32 +818,42 @@ __global__ __launch_bounds__(UPD_NT) void k_update(TF *__restrict__ x, const TA 
     double sum = 0.0;
     unsigned long long nneg = 0;
     unsigned int bad = 0;
-    unsigned long long badidx = ~0ull;
+    unsigned long long badkey = ~0ull;
     const TA thr = sizeof(TA) == 4 ? (TA)1e-8f : (TA)1e-8;
     const TA neg = sizeof(TA) == 4 ? (TA)(-1.0e-6f) : (TA)(-1.0e-6);
-    for (size_t t = (size_t)blockIdx.x * UPD_NT + threadIdx.x; t < nout; t += (size_t)UPD_BLOCKS * UPD_NT) {
+    // the detector stage of this iteration, or an earlier iteration, failed:
+    // the reference never reaches this update, so x stays as it is
+    const bool hold = idiag->halt || (idiag->err & (UPDC_E_MU | UPDC_E_RATIO));
+    for (size_t t = (size_t)blockIdx.x * UPD_NT + threadIdx.x; t < nout && !hold; t += (size_t)UPD_BLOCKS * UPD_NT) {
         const TA d = D[t];
         TF xv = x[t];
+        unsigned int b = 0;
         if (d > thr) {
             const TA c = C[t];
             TA f = c / d;
-            if (!isfinite(f)) { bad |= UPDC_E_FACTOR; badidx = badidx < t ? badidx : t; }
+            if (!isfinite(f)) b = b ? b : UPDC_E_FACTOR;
             if (f < (TA)0) {
-                if (c < neg * max_sens) { bad |= UPDC_E_NEGCORR; badidx = badidx < t ? badidx : t; }
+                if (c < neg * max_sens) b = b ? b : UPDC_E_NEGCORR;
                 f = (TA)0;
             }
             xv = (TF)((TA)xv * f);
         } else {
             xv = (TF)0;
         }
-        if (!isfinite(xv)) { bad |= UPDC_E_ESTIMATE; badidx = badidx < t ? badidx : t; }
+        if (!isfinite(xv)) b = b ? b : UPDC_E_ESTIMATE;
+        if (b) {
+            bad |= b;
+            const unsigned long long key =
+                err_key_update(t, b == UPDC_E_FACTOR ? 0u : b == UPDC_E_NEGCORR ? 1u : 2u);
+            badkey = badkey < key ? badkey : key;
+        }
         x[t] = xv;
         sum += (double)xv;
         nneg += xv < (TF)0 ? 1 : 0;
     }
     if (bad) {
         atomicOr(&idiag->err, bad);
-        atomicMin(&idiag->err_index, badidx);
+        atomicMin(&idiag->err_key, badkey);
     }
     sh[threadIdx.x] = sum;
     shn[threadIdx.x] = nneg;
@@ -958,8 +972,8 @@ __global__ void k_init_diag(ExpDiag *d, int n, IterDiag *it, int nit) {
         it[i].signed_sum = 0.0;
         it[i].n_negative = 0;
         it[i].err = 0;
-        it[i].pad = 0;
-        it[i].err_index = ~0ull;
+        it[i].halt = 0;
+        it[i].err_key = ~0ull;
     }
 }
 hipError_t init_diag(ExpDiag *d, int n, IterDiag *it, int nit, hipStream_t st) {
@@ -985,10 +999,11 @@ __global__ void k_commit_diag(ExpDiag *cur, int nexp, IterDiag *icur, ExpDiag *h
         cur[k].min_key = ~0ull;
     }
     if (i == 0) {
+        if (icur->err) icur->halt = 1u;  // sticky for the rest of the call
         icur->signed_sum = 0.0;
         icur->n_negative = 0;
         icur->err = 0;
-        icur->err_index = ~0ull;
+        icur->err_key = ~0ull;
         *ctr = it + 1;
     }
 }

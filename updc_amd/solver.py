@@ -34,6 +34,15 @@ def _pd(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+class IterationError(_lib.UpdcError):
+    """updc_iterate failed; freeze/info hold the rows written before the stop
+    and completed the iterations whose update ran."""
+
+    def __init__(self, code: int, msg: str, freeze: list, info: list, completed: int):
+        super().__init__(code, msg)
+        self.freeze, self.info, self.completed = freeze, info, completed
+
+
 @dataclass
 class Exposure:
     """One manifest row's data (updc_core.c:426-461) as arrays."""
@@ -165,13 +174,23 @@ class Context:
 
     # -- iteration
     def iterate(self, n: int = 1):
+        """Run n iterations; returns (freeze rows, per-iteration info).  A
+        numeric guard raises IterationError carrying what the reference would
+        have written before stopping (updc_core.c:1410-1479)."""
         rows = (FreezeRow * (n * self.nexp))()
         info = (IterInfo * n)()
-        check(self.lib.updc_iterate(self.h, int(n), rows, info))
-        freeze = [(r.iteration, r.exposure, r.n_new_frozen, r.n_total_frozen, r.min_raw_mu) for r in rows]
+        rc = self.lib.updc_iterate(self.h, int(n), rows, info)
+        msg = self.lib.updc_last_error().decode(errors="replace") if rc != _lib.UPDC_OK else ""
+        done, with_rows = C.c_int(n), C.c_int(n)
+        if rc != _lib.UPDC_OK:
+            check(self.lib.updc_get_iterate_progress(self.h, C.byref(done), C.byref(with_rows)))
+        freeze = [(r.iteration, r.exposure, r.n_new_frozen, r.n_total_frozen, r.min_raw_mu)
+                  for r in rows[:with_rows.value * self.nexp]]
         its = [dict(iteration=i.iteration, signed_sum=i.signed_sum, n_negative=i.n_negative,
                     n_new_frozen=i.n_new_frozen, n_total_frozen=i.n_total_frozen, min_raw_mu=i.min_raw_mu,
-                    ms=i.ms_total) for i in info]
+                    ms=i.ms_total) for i in info[:done.value]]
+        if rc != _lib.UPDC_OK:
+            raise IterationError(rc, msg, freeze, its, done.value)
         return freeze, its
 
     # -- products
@@ -273,6 +292,16 @@ class Context:
         vv = _f64(v).reshape(-1)
         out = np.empty((self.H, self.W), np.float64)
         check(self.lib.updc_stage_scatter(self.h, k, _pd(vv), _pd(out)))
+        return out
+
+    def adjoint_images(self, v: np.ndarray) -> np.ndarray:
+        """S_k^T v_k for every exposure through the iteration's adjoint-image
+        launch; v is the flattened per-pixel vector (sum of nin)."""
+        vv = _f64(v).reshape(-1)
+        if vv.size != sum(self.nin):
+            raise ValueError("per-pixel vector has the wrong length")
+        out = np.empty((self.nexp, self.H, self.W), np.float64)
+        check(self.lib.updc_stage_adjoint_images(self.h, _pd(vv), _pd(out)))
         return out
 
     def fft_r2c(self, image: np.ndarray) -> np.ndarray:
