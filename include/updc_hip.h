@@ -289,6 +289,20 @@ int updc_op_r2c(updc_ctx *ctx, const void *image, int rows, int cols, int pitch,
 /* image = scale * C2R(spectrum), cropped to rows x cols (row pitch `pitch`). */
 int updc_op_c2r(updc_ctx *ctx, const void *spectrum, void *image, int rows, int cols, int pitch, double scale,
                 void *stream);
+/* Allocation-free variant of updc_op_c2r, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  Same arguments, kernels and results (bit for
+ * bit) as updc_op_c2r; its column-inverse workspace is a static device array of
+ * UPDC_NOALLOC_WORKSPACE_BYTES (128 MiB) that every GPU loading the library
+ * holds until the process ends, instead of a buffer the context allocates on
+ * its first updc_op_c2r call.  The spectrum's workspace, (FW/2 + 1) * pitch
+ * elements of 8 (F32, MIXED) or 16 (F64) bytes, must fit: an FP32 5600 x 5600
+ * transform does (125.5 MB), an FP64 one does not (UPDC_ERR_ARG, nothing
+ * launched).  Every call on one GPU shares that array, whatever the context:
+ * calls that overlap in time on one GPU (other streams, other threads) race. */
+#define UPDC_NOALLOC_WORKSPACE_BYTES 134217728u
+int updc_op_c2r_noalloc(updc_ctx *ctx, const void *spectrum, void *image, int rows, int cols, int pitch,
+                        double scale, void *stream);
 /* out[p] = sum_z frac_z image[t_z] for the nin_k pixels of exposure k (S_k,
  * the fmaf chain of updc_core.c:1154-1158); image is H x W. */
 int updc_op_gather(updc_ctx *ctx, int k, const void *image, void *out, void *stream);

@@ -66,6 +66,7 @@ SIGNATURES = {
     "updc_op_layout": (C.c_int, [_P, _I, _I, _I, _I, _I]),
     "updc_op_r2c": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "updc_op_c2r": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, _P]),
+    "updc_op_c2r_noalloc": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, _P]),
     "updc_op_gather": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "updc_op_scatter": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "updc_add_exposure": (C.c_int, [_P, _F, _F, C.c_int, C.c_int, _P, C.c_size_t, _F, C.c_int, C.c_int, C.c_int,

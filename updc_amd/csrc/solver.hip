@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -235,6 +236,11 @@ struct PipeBase {
     virtual void ensure_plain_buffers(Ctx &c) = 0;
     // device-pointer operators (element type TF), on the caller's stream
     virtual void op_r2c(Ctx &c, const void *img, int rows, int cols, int pitch, void *spec, hipStream_t st) = 0;
+    // op_c2r with a caller-provided column-inverse workspace (device memory,
+    // op_c2r_workspace_bytes() of it); allocates nothing
+    virtual void op_c2r_ws(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double scale,
+                           void *ws, hipStream_t st) = 0;
+    virtual size_t op_c2r_workspace_bytes() const = 0;
     virtual void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double scale,
                         hipStream_t st) = 0;
     virtual void op_gather(Ctx &c, int k, const void *img, void *out, hipStream_t st) = 0;
@@ -1117,14 +1123,21 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(fft.row_r2c(ROW_PLAIN, (const TF *)img, 0, pitch, cols, nullptr, nullptr, 0, sp, 0, s.FH, rows, 1, st));
         HIPCHK(fft.template colpass<TA>(COL_FWD, sp, 0, s.FH, sp, 0, s.FH, nullptr, 0, rows, s.FH, 1, st));
     }
-    void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double sc,
-                hipStream_t st) override {
+    size_t op_c2r_workspace_bytes() const override { return fft.spec_elems(fft.s.FH) * sizeof(Cx<TF>); }
+    void op_c2r_ws(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double sc, void *ws,
+                   hipStream_t st) override {
         (void)c;
         const FftShape &s = fft.s;
-        if (!op_ws.p) op_ws.alloc(fft.spec_elems(s.FH));
-        HIPCHK(fft.template colpass<TA>(COL_INV, (const Cx<TF> *)spec, 0, s.FH, op_ws.p, 0, s.FH, nullptr, 0, s.FH,
-                                        rows, 1, st));
-        HIPCHK(fft.row_c2r(op_ws.p, 0, s.FH, (TF *)img, 0, pitch, cols, (TF)sc, rows, 1, st));
+        Cx<TF> *w = (Cx<TF> *)ws;
+        // the column inverse writes rows [0, rows) of every column, all the row pass reads
+        HIPCHK(fft.template colpass<TA>(COL_INV, (const Cx<TF> *)spec, 0, s.FH, w, 0, s.FH, nullptr, 0, s.FH, rows,
+                                        1, st));
+        HIPCHK(fft.row_c2r(w, 0, s.FH, (TF *)img, 0, pitch, cols, (TF)sc, rows, 1, st));
+    }
+    void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double sc,
+                hipStream_t st) override {
+        if (!op_ws.p) op_ws.alloc(fft.spec_elems(fft.s.FH));
+        op_c2r_ws(c, spec, img, rows, cols, pitch, sc, op_ws.p, st);
     }
     void op_gather(Ctx &c, int k, const void *img, void *out, hipStream_t st) override {
         // the kernels index per-pixel arrays at pix_k + p: shift the caller's
@@ -1499,6 +1512,8 @@ int updc_self_test(int verbose) {
     return 0;
 }
 
+static void *scratch_on(int dev);
+
 int updc_ctx_create(updc_ctx **out, int width, int height, int fft_width, int fft_height, int precision,
                     unsigned flags, int device) {
     return guarded([&] {
@@ -1512,6 +1527,7 @@ int updc_ctx_create(updc_ctx **out, int width, int height, int fft_width, int ff
         HIPCHK(hipGetDeviceCount(&ndev));
         need(device >= 0 && device < ndev, UPDC_ERR_HIP, "no such HIP device");
         HIPCHK(hipSetDevice(device));
+        (void)scratch_on(device);  // the allocation-free entry point's workspace address on this device
         auto c = std::make_unique<updc_ctx>();
         c->W = width;
         c->H = height;
@@ -1959,6 +1975,45 @@ int updc_op_c2r(updc_ctx *ctx, const void *spectrum, void *image, int rows, int 
              UPDC_ERR_ARG, "invalid operator arguments");
         HIPCHK(hipSetDevice(ctx->device));
         ctx->pipe->op_c2r(*ctx, spectrum, image, rows, cols, pitch, scale, (hipStream_t)stream);
+    });
+}
+// ---- allocation-free C2R (updc_op_c2r_noalloc)
+// Its column-inverse workspace: a file-scope device array, one copy on each
+// GPU that loads the library, held until the process ends.  Host code never
+// passes the array's name to a kernel or a copy (on the host it is only a
+// stand-in); it uses the per-device address hipGetSymbolAddress returns.
+static __device__ __align__(256) unsigned char scratch[UPDC_NOALLOC_WORKSPACE_BYTES];
+static constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Address of `scratch` on the current device `dev` (looked up once per device).
+static void *scratch_on(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw Error(UPDC_ERR_ARG, "device ordinal beyond the workspace table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(UPDC_ERR_HIP, "no device address for the allocation-free workspace");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+int updc_op_c2r_noalloc(updc_ctx *ctx, const void *spectrum, void *image, int rows, int cols, int pitch,
+                        double scale, void *stream) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(image && spectrum && rows >= 1 && rows <= ctx->FH && cols >= 1 && cols <= ctx->FW && pitch >= cols,
+             UPDC_ERR_ARG, "invalid operator arguments");
+        const size_t need_bytes = ctx->pipe->op_c2r_workspace_bytes();
+        if (need_bytes > (size_t)UPDC_NOALLOC_WORKSPACE_BYTES)
+            throw Error(UPDC_ERR_ARG, "spectrum of " + std::to_string(need_bytes) +
+                                          " bytes exceeds the allocation-free workspace of " +
+                                          std::to_string((size_t)UPDC_NOALLOC_WORKSPACE_BYTES) + " bytes");
+        HIPCHK(hipSetDevice(ctx->device));
+        void *ws = scratch_on(ctx->device);
+        ctx->pipe->op_c2r_ws(*ctx, spectrum, image, rows, cols, pitch, scale, ws, (hipStream_t)stream);
     });
 }
 int updc_op_gather(updc_ctx *ctx, int k, const void *image, void *out, void *stream) {
