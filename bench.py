@@ -13,7 +13,12 @@ Reported line: iterations/s of the FP32/batched form (FP32 FFTs, PSF and S_k
 products, FP64 spectrum accumulation over exposures, denominator, raw_mu test
 and ratio; all exposures batched in every launch; each iteration replayed as
 a hipGraph), next to the plain FP64 GPU form and the error of the FP32/batched
-estimate against the FP64 one after the same number of iterations.
+estimate against the FP64 one after the same number of iterations.  The
+roofline block gives the dominant kernel against the HBM peak (its bytes on
+this library's data layout and on the LinkMap unit) and the whole iteration's
+algorithmic bytes against it.  The CPU baseline is the reference's float32
+iteration on the host's cores (all of them, then one), MKL transforms in place
+of FFTW3f.
 
 Usage: python bench.py [--gpus 1] [--steps 40] [--warmup 2] [--config 2|1|3|4|5]
                        [--no-cpu] [--no-f64] [--no-autotune] [--profile]
@@ -370,24 +375,33 @@ def stage_profile(c, iters: int) -> dict:
     return t
 
 
-def cpu_baseline(ds, fft_shape, iters: int = 2) -> dict:
-    """The oracle's float32 restatement (numpy pocketfft + C detector loops,
-    single thread) timed on this host: `iters` full iterations over every
-    exposure of the same workload, after its (untimed) set-up."""
+def cpu_baseline(ds, fft_shape, iters_all: int = 2, iters_one: int = 1) -> dict:
+    """The reference's float32 iteration on this host's CPU cores
+    (oracle/cpu_baseline.py: the oracle's C detector and spectrum loops on
+    exposure threads, worker w owning exposures k = w mod workers as
+    updc_core.c:1201-1269 does, MKL transforms through torch.fft because
+    FFTW3f is absent), after its untimed set-up: `iters_all` iterations on all
+    the cores this process may use (at most 16), then `iters_one` on one core."""
+    from oracle import cpu_baseline as B
     from oracle import oracle as O
     ex = [dict(sci=ds.sci[k], wht=ds.wht[k], psf=ds.psfs[k], background=ds.background[k],
                pedestal=ds.pedestal[k], linkmap=ds.linkmaps[k]) for k in range(len(ds.geoms))]
+    cores = B.host_cores()
     t0 = time.perf_counter()
     s = O.RefSolver(ds.W, ds.H, ex, "f32", fft_shape=fft_shape)
-    log(f"cpu oracle: set-up {time.perf_counter() - t0:.1f}s")
-    t0 = time.perf_counter()
-    s.iterate(iters)
-    t = (time.perf_counter() - t0) / iters
-    log(f"cpu oracle: {iters} iterations, {t:.2f}s per iteration")
-    return dict(value=1.0 / t, unit="iters/s", cores=1, kind="port",
-                sample=f"{iters} iterations of the oracle's f32 path over all {len(ex)} exposures of this workload "
-                       f"(FFT {fft_shape[0]}x{fft_shape[1]}), after its untimed set-up; {t:.2f}s per iteration; "
-                       f"numpy pocketfft (one thread) in place of FFTW3f")
+    log(f"cpu baseline: oracle set-up {time.perf_counter() - t0:.1f}s")
+    t_all = B.time_iterations(s, cores, cores, iters_all)
+    log(f"cpu baseline: {cores} cores, {t_all:.2f}s per iteration")
+    t_one = B.time_iterations(s, 1, 1, iters_one)
+    log(f"cpu baseline: 1 core, {t_one:.2f}s per iteration")
+    return dict(value=1.0 / t_all, unit="iters/s", cores=cores, kind="port",
+                sample=f"{iters_all} iterations on {cores} cores and {iters_one} on 1 core of the reference's "
+                       f"float32 iteration over all {len(ex)} exposures of this workload (FFT {fft_shape[0]}x"
+                       f"{fft_shape[1]}), after the untimed set-up; {t_all:.2f}s and {t_one:.2f}s per iteration",
+                single_core=dict(value=1.0 / t_one, unit="iters/s", cores=1),
+                host=dict(model=B.cpu_model(), nproc=os.cpu_count(), cores_used=cores),
+                fft="MKL through torch.fft on the CPU (FFTW3f, the reference's library, is absent)",
+                threading="exposure workers = FFT threads = cores (updc_core.c:159-168, 1201-1269)")
 
 
 def main() -> int:
@@ -501,6 +515,20 @@ def main() -> int:
     per_stage = {k: dict(ms=round(v[0], 4), GBps=round(sb[k] / (v[0] * 1e-3) / 1e9, 1)) for k, v in kern.items()}
     log(f"per-kernel: {json.dumps(per_stage)}")
 
+    # the whole iteration against the byte model of the path (each stage's
+    # operands read once, results written once; real LinkMap sizes)
+    nin_all = [i["nin"] for i in infos]
+    lm_all = [i["linkmap_bytes"] for i in infos]
+    b_iter = metrics.iteration_bytes(ds.W, ds.H, FW, FH, nin_all, lm_all)
+    it_gbs = b_iter * its32 / 1e9
+    # the dominant kernel on the LinkMap unit too: the adjoint gathers read the
+    # CSC transpose this library builds, the reference reads the LinkMap
+    link_unit = None
+    if dom in ("adjoint_image", "row_r2c_gather"):
+        csc_b = sum(4 * (ds.W * ds.H + 1) + 8 * i["ncsc"] for i in infos)
+        lu_bytes = sb[dom] - csc_b + sum(lm_all)
+        link_unit = dict(bytes=lu_bytes, frac=round(lu_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+
     # HBM traffic of the dominant kernel, from the committed FETCH_SIZE /
     # WRITE_SIZE summary of this workload (rocprofv3 --pmc runs; see
     # updc_amd/profsum.py for the copy-kernel calibration)
@@ -540,7 +568,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "algorithmic_bytes_per_launch": sb[dom], "avg_launch_ms": round(ms, 4),
-                     "traffic_source": tpath.name if traffic is not None else None},
+                     "traffic_source": tpath.name if traffic is not None else None,
+                     "linkmap_unit": link_unit,
+                     "iteration_bytes": b_iter, "iteration_achieved": round(it_gbs, 1),
+                     "iteration_frac": round(it_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

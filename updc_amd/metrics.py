@@ -73,3 +73,31 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
             "row_c2r_update": acc_h + nout * (2 * e + e_d),
         })
     return out
+
+
+def iteration_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], lm_bytes: list[int], e: int = 4,
+                    c: int = 8) -> int:
+    """Algorithmic HBM bytes of one whole iteration, stage by stage, each
+    stage reading its operands once and writing its results once (FFT-internal
+    passes excluded), with the real LinkMap sizes:
+
+      S1 X^ = R2C(x)                       nout e + nc c
+      S2 blur_k = crop C2R(X^ P^_k)        N (2 nc c + nout e)
+      S3 detector (LinkMap, sci, w,        sum_k (LM_k + 10 nin_k) + N 2 nout e
+         frozen r/w; blur in, scatter out)
+      S4 sum_k conj(P^_k) R2C(scatter_k)   N (nout e + nc c) + nc c
+      S5 C = crop C2R(sum)                 nc c + nout e
+      S6 update (x, C, D in; x out)        4 nout e
+
+    nc = FH (FW/2 + 1) complex elements; e/c the real/complex element bytes
+    (4/8 for float32 storage)."""
+    N = len(nin)
+    nout = W * H
+    nc = FH * (FW // 2 + 1)
+    s1 = nout * e + nc * c
+    s2 = N * (2 * nc * c + nout * e)
+    s3 = sum(lm + 10 * n for lm, n in zip(lm_bytes, nin)) + N * 2 * nout * e
+    s4 = N * (nout * e + nc * c) + nc * c
+    s5 = nc * c + nout * e
+    s6 = 4 * nout * e
+    return s1 + s2 + s3 + s4 + s5 + s6
