@@ -507,7 +507,8 @@ def main() -> int:
     e, c, ed, ea = 4, 8, 8, 8
     sb = metrics.stage_bytes(ds.W, ds.H, FW, FH, [i["nin"] for i in infos], [i["nlinks"] for i in infos],
                              [i["ncsc"] for i in infos], [i["linkmap_bytes"] for i in infos], e, c, ed, ea,
-                             fused_inverse="col_inv_corr" not in prof)
+                             fused_inverse="col_inv_corr" not in prof,
+                             adj_csc_bytes=[i["csc_bytes"] for i in infos])
     kern = {k: v for k, v in prof.items() if k in sb}
     dom = max(kern, key=lambda k: kern[k][0])
     ms = kern[dom][0]
@@ -525,7 +526,8 @@ def main() -> int:
     # CSC transpose this library builds, the reference reads the LinkMap
     link_unit = None
     if dom in ("adjoint_image", "row_r2c_gather"):
-        csc_b = sum(4 * (ds.W * ds.H + 1) + 8 * i["ncsc"] for i in infos)
+        csc_b = sum(i["csc_bytes"] for i in infos) if dom == "adjoint_image" else \
+            sum(4 * (ds.W * ds.H + 1) + 8 * i["ncsc"] for i in infos)
         lu_bytes = sb[dom] - csc_b + sum(lm_all)
         link_unit = dict(bytes=lu_bytes, frac=round(lu_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
 

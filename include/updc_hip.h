@@ -229,6 +229,13 @@ int updc_get_frozen(updc_ctx *ctx, int k, uint8_t *out);
 int updc_get_exposure_info(updc_ctx *ctx, int k, uint64_t *nin, uint64_t *nlinks, uint32_t *encoding,
                            uint64_t *ncsc);
 
+/* The transposed LinkMap of exposure k that the iteration's adjoint gather
+ * reads: compact = 1 for the 6-byte-per-link form (every 64-target tile has
+ * at most 255 links per target and spans at most 65536 source pixels), 0 for
+ * the 8-byte form; bytes = its size.  No reference counterpart (the
+ * reference scatters, updc_core.c:1171-1176). */
+int updc_get_csc_layout(updc_ctx *ctx, int k, int *compact, uint64_t *bytes);
+
 /* Stage entry points (parity tests, autograd).  All host buffers, f64 at the
  * boundary, computed in the context's precision on the GPU. */
 /* blur_k = crop(C2R(X^ . P^_k)) / nf for every exposure: out is N*H*W. */

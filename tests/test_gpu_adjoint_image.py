@@ -13,6 +13,12 @@ same detector loop), the GPU images must equal the oracle's scatter exactly:
   past the batched entries);
 - ABS32 LinkMaps (the CSC built from 32-bit target words);
 - FreezeMu firing (wr = w on frozen pixels) and masked pixels (w = 0).
+
+The gather reads a compact copy of the transposed map (16-bit source pixel
+offsets per 64-target tile, 8-bit counts) wherever it fits, and the 8-byte
+form elsewhere: a detector so fine that targets collect more than 255 links,
+and a wide detector turned 90 degrees, whose target rows run down detector
+columns so a tile spans more than 65535 source pixels, take the second path.
 """
 import numpy as np
 import pytest
@@ -41,13 +47,14 @@ def _oracle_wr_and_scatter(ds):
     return np.concatenate(wrs), np.stack(scs)
 
 
-def _gpu_images(ds, prec, linkmaps, wr):
+def _gpu_images(ds, prec, linkmaps, wr, compact=True):
     from updc_amd.solver import Context, Exposure
     with Context(ds.W, ds.H, prec) as c:
         for k in range(len(ds.geoms)):
             c.add_exposure(Exposure(ds.sci[k], ds.wht[k], ds.psfs[k], ds.background[k], ds.pedestal[k],
                                     linkmap=linkmaps[k]))
         c.prepare()
+        assert all(c.exposure_info(k)["csc_compact"] == compact for k in range(len(ds.geoms)))
         return c.adjoint_images(wr)
 
 
@@ -75,3 +82,32 @@ def test_adjoint_images_bitexact(small_ds, fine_ds, case, prec):
         f"{int((got.astype(np.float32) != want).sum())} pixels differ"
     # and the images were really exercised: non-zero on every exposure
     assert all(np.count_nonzero(want[k]) > 100 for k in range(len(ds.geoms)))
+
+
+def _custom(W, H, geoms, seed):
+    from updc_amd import synth
+    g = np.random.Generator(np.random.PCG64(seed))
+    truth = synth.truth_image(W, H, 10, 0.5, g, sigma_range=(0.6, 2.0))
+    psfs = [synth.gaussian_psf(5, 2.0) for _ in geoms]
+    return oracle_dataset(synth.Dataset(W, H, geoms, psfs, truth, [0.0] * len(geoms), 2.0, 0.02, None, 5, "custom"))
+
+
+@pytest.mark.parametrize("case", ["very_fine", "turned_wide"])
+def test_adjoint_images_bitexact_on_the_wide_form(case):
+    from oracle import oracle as O
+    from updc_amd import synth
+    print(f"case: adjoint images, 8-byte form ({case})", flush=True)
+    if case == "very_fine":
+        # 20 detector pixels per target pixel along each axis: ~400 links per target
+        ds = _custom(16, 16, [synth.Geometry(320, 320, 0.05, 1.0, (0.3, 0.2), None, (7.5, 7.5))], 3)
+        p = O.parse_linkmap(ds.linkmaps[0])
+        assert np.bincount(p["target"]).max() > 255
+    else:
+        # 1800 x 40 detector turned 90 degrees onto a 64-wide target: a target row
+        # crosses 40 detector rows 1800 pixels apart
+        ds = _custom(64, 1808, [synth.Geometry(1800, 40, 1.0, 90.0, (0.25, 0.1), None, (31.5, 903.5))], 4)
+    wr, want = _oracle_wr_and_scatter(ds)
+    for prec in ("f32", "mixed"):
+        got = _gpu_images(ds, prec, ds.linkmaps, wr.astype(np.float64), compact=False)
+        assert np.array_equal(got.astype(np.float32), want)
+    assert np.count_nonzero(want) > 100

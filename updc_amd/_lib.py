@@ -89,6 +89,7 @@ SIGNATURES = {
     "updc_get_frozen": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint8)]),
     "updc_get_exposure_info": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "updc_get_csc_layout": (C.c_int, [_P, C.c_int, _I, C.POINTER(C.c_uint64)]),
     "updc_stage_forward": (C.c_int, [_P, _D, _D]),
     "updc_stage_model": (C.c_int, [_P, _D, _D]),
     "updc_stage_detector": (C.c_int, [_P, C.c_int, _D, _D, _D, _D]),

@@ -177,6 +177,15 @@ struct HostExp {
     DevBuf<uint32_t> toff;
     DevBuf<uint2> tent;
     uint64_t ncsc = 0;
+    // the compact copy of the CSC the iteration's adjoint gather reads (c16)
+    bool c16 = false;
+    DevBuf<uint32_t> c_toff, c_pb;
+    DevBuf<uint8_t> c_cnt;
+    DevBuf<uint16_t> c_dp;
+    DevBuf<float> c_fr;
+    size_t csc_bytes(size_t nout) const {
+        return c16 ? ((nout + 63) / 64 + 1) * 4 + (nout + 63) / 64 * 4 + nout + ncsc * 6 : (nout + 1) * 4 + ncsc * 8;
+    }
     ExpDev view() const {
         ExpDev e{};
         e.nin = (uint32_t)nin;
@@ -197,6 +206,12 @@ struct HostExp {
         e.frac = (const float *)(b + pos);
         e.toff = toff.p;
         e.tent = tent.p;
+        e.c16 = c16 ? 1u : 0u;
+        e.c_toff = c_toff.p;
+        e.c_pb = c_pb.p;
+        e.c_cnt = c_cnt.p;
+        e.c_dp = c_dp.p;
+        e.c_fr = c_fr.p;
         e.pix = pix;
         e.background = background;
         e.pedestal = pedestal;
@@ -326,6 +341,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     DevBuf<Cx<TA>> spec_xa;   // kWide: the sum after its TA column inverse, rows [0, H)
     bool acc_fused = false;   // kWide: that inverse runs inside the accumulating column pass
     bool fused_update = false;  // the final row C2R applies the update (persistent row pass available)
+    bool any_c16 = false, any_wide = false;  // exposures with / without the compact CSC
     DevBuf<ExpDiag> diag_cur, diag_hist;
     DevBuf<IterDiag> idiag_cur, idiag_hist;
     DevBuf<double> partials;
@@ -466,8 +482,31 @@ template <class TF, class TA> struct Pipe : PipeBase {
             HIPCHK(hipMemcpyAsync(one.p, &v, sizeof(ExpDev), hipMemcpyHostToDevice, c.st));
             HIPCHK(csc_fill<TF>(one.p, (uint32_t)e.nin, w.p + e.pix, cursor.p, e.tent.p, c.st));
             HIPCHK(csc_sort(e.toff.p, e.tent.p, nout, c.st));
+            // the compact copy for the iteration's adjoint gather, where it fits
+            const size_t ntiles = (nout + 63) / 64;
+            e.c_toff.alloc(ntiles + 1);
+            e.c_pb.alloc(ntiles);
+            e.c_cnt.alloc(nout);
+            e.c_dp.alloc(std::max<uint64_t>(total, 1));
+            e.c_fr.alloc(std::max<uint64_t>(total, 1));
+            HIPCHK(fill<unsigned int>(err.p, 0u, 1, c.st));
+            HIPCHK(csc16_build(e.toff.p, e.tent.p, nout, e.c_toff.p, e.c_pb.p, e.c_cnt.p, e.c_dp.p, e.c_fr.p, err.p,
+                               c.st));
+            unsigned int fl = 0;
+            HIPCHK(hipMemcpyAsync(&fl, err.p, 4, hipMemcpyDeviceToHost, c.st));
             HIPCHK(hipStreamSynchronize(c.st));
+            e.c16 = fl == 0;
+            if (!e.c16) {
+                e.c_toff.reset();
+                e.c_pb.reset();
+                e.c_cnt.reset();
+                e.c_dp.reset();
+                e.c_fr.reset();
+            }
         }
+        HIPCHK(fill<unsigned int>(err.p, 0u, 1, c.st));
+        any_c16 = any_wide = false;
+        for (auto &x : c.ex) (x->c16 ? any_c16 : any_wide) = true;
     }
 
     void upload_ex(Ctx &c) {
@@ -759,7 +798,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             // the CU with the transform tile at 4, and the fused launch is slower
             // than the two (1.58 ms against 0.91 + 0.60 ms with 8 exposures of
             // 2048² on a 4096² grid)
-            HIPCHK(csc_image<TF>(d_ex.p, n, wr.p, nout, blur.p, c.st));
+            HIPCHK(csc_image<TF>(d_ex.p, n, wr.p, nout, blur.p, any_c16, any_wide, c.st));
             mark(m++);
             HIPCHK(fft.row_r2c(ROW_PLAIN, blur.p, nout, c.W, c.W, nullptr, nullptr, 0, spec_k.p, kstride, c.H, c.H, n,
                                c.st));
@@ -1070,7 +1109,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     void stage_adjoint_images(Ctx &c, const double *v, double *out) override {
         const size_t nout = c.nout();
         from_host_f64(v, c.npix(), wr.p, c);
-        HIPCHK(csc_image<TF>(d_ex.p, N(c), wr.p, nout, blur.p, c.st));
+        HIPCHK(csc_image<TF>(d_ex.p, N(c), wr.p, nout, blur.p, any_c16, any_wide, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(blur.p, (size_t)N(c) * nout, out, c);
     }
@@ -1909,6 +1948,15 @@ int updc_get_exposure_info(updc_ctx *ctx, int k, uint64_t *nin, uint64_t *nlinks
         if (nlinks) *nlinks = e.nlinks;
         if (encoding) *encoding = e.encoding;
         if (ncsc) *ncsc = e.ncsc;
+    });
+}
+int updc_get_csc_layout(updc_ctx *ctx, int k, int *compact, uint64_t *bytes) {
+    return guarded([&] {
+        NEED_PREPARED(ctx);
+        need(k >= 0 && k < (int)ctx->ex.size(), UPDC_ERR_ARG, "invalid exposure index");
+        const HostExp &e = *ctx->ex[k];
+        if (compact) *compact = e.c16 ? 1 : 0;
+        if (bytes) *bytes = e.csc_bytes(ctx->nout());
     });
 }
 int updc_reset_frozen(updc_ctx *ctx) {

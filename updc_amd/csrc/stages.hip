@@ -430,6 +430,54 @@ __global__ __launch_bounds__(256) void k_csc_sort(const uint32_t *__restrict__ t
         tent[j] = key;
     }
 }
+// The compact copy of a sorted CSC, one wave per 64-target tile: the tile's
+// first entry and smallest source pixel, each target's count, each entry's
+// pixel minus that smallest pixel and its fraction (same order, so a gather
+// over it repeats the ascending-p chain).  flags bit 0: a target with more
+// than 255 entries; bit 1: a tile whose pixels span more than 16 bits.
+__global__ __launch_bounds__(256) void k_csc16_build(const uint32_t *__restrict__ toff, const uint2 *__restrict__ tent,
+                                                     size_t nout, uint32_t *__restrict__ c_toff,
+                                                     uint32_t *__restrict__ c_pb, uint8_t *__restrict__ c_cnt,
+                                                     uint16_t *__restrict__ c_dp, float *__restrict__ c_fr,
+                                                     unsigned int *flags) {
+    const size_t ntiles = (nout + 63) / 64;
+    const size_t tile = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;  // uniform across the wave
+    const int lane = threadIdx.x & 63;
+    const size_t t = tile * 64 + lane;
+    uint32_t a = 0, b = 0;
+    if (t < nout) {
+        a = toff[t];
+        b = toff[t + 1];
+    }
+    // entries of a target are sorted by pixel: first and last bound its range
+    uint32_t lo = b > a ? tent[a].x : 0xffffffffu, hi = b > a ? tent[b - 1].x : 0u;
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+    }
+    const uint32_t base = lo == 0xffffffffu ? 0u : lo;
+    if (lane == 0) {
+        c_toff[tile] = toff[tile * 64];
+        c_pb[tile] = base;
+        if (tile == ntiles - 1) c_toff[ntiles] = toff[nout];
+        if (lo != 0xffffffffu && hi - lo > 0xffffu) atomicOr(flags, 2u);
+    }
+    if (t < nout) c_cnt[t] = (uint8_t)min(b - a, 255u);
+    if (b - a > 255u) atomicOr(flags, 1u);
+    for (uint32_t i = a; i < b; ++i) {
+        const uint2 en = tent[i];
+        c_dp[i] = (uint16_t)(en.x - base);
+        c_fr[i] = __uint_as_float(en.y);
+    }
+}
+hipError_t csc16_build(const uint32_t *toff, const uint2 *tent, size_t nout, uint32_t *c_toff, uint32_t *c_pb,
+                       uint8_t *c_cnt, uint16_t *c_dp, float *c_fr, unsigned int *flags, hipStream_t st) {
+    const size_t ntiles = (nout + 63) / 64;
+    hipLaunchKernelGGL(k_csc16_build, dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, st, toff, tent, nout, c_toff,
+                       c_pb, c_cnt, c_dp, c_fr, flags);
+    return hipGetLastError();
+}
 template <class TW>
 hipError_t csc_count(const ExpDev *e_dev, uint32_t nin, const TW *w, uint32_t *tcnt, hipStream_t st) {
     hipLaunchKernelGGL((k_csc_count<TW>), dim3((nin + 255) / 256), dim3(256), 0, st, e_dev, nin, w, tcnt);
@@ -1053,9 +1101,10 @@ static constexpr int CSCI_NT = 256, CSCI_U = 4;
 template <class TF> constexpr int csci_g() { return sizeof(TF) == 4 ? 3 : 2; }
 template <class TF>
 __global__ __launch_bounds__(CSCI_NT) void k_csc_image(const ExpDev *__restrict__ ex, const TF *__restrict__ v,
-                                                       size_t nout, TF *__restrict__ out) {
+                                                       size_t nout, TF *__restrict__ out, int skip_c16) {
     constexpr int G = csci_g<TF>();
     const ExpDev &e = ex[blockIdx.y];
+    if (skip_c16 && e.c16) return;  // that exposure runs in k_csc_image16
     const TF *vk = v + e.pix;
     TF *ok = out + (size_t)blockIdx.y * nout;
     // XCD-aware: the blocks of one XCD (blockIdx.x % 8; the host makes
@@ -1099,15 +1148,92 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image(const ExpDev *__restrict_
         }
     }
 }
-template <class TF> hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, hipStream_t st) {
+// The same gather over the compact CSC (exposures with c16 set).  A wave
+// covers one 64-target tile per target group: the tile's first entry and
+// pixel base are wave-uniform, each lane's first entry is that plus the
+// exclusive prefix sum of the counts of the lanes before it.
+template <class TF>
+__global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restrict__ ex, const TF *__restrict__ v,
+                                                         size_t nout, TF *__restrict__ out) {
+    constexpr int G = csci_g<TF>();
+    const ExpDev &e = ex[blockIdx.y];
+    if (!e.c16) return;
+    const TF *vk = v + e.pix;
+    TF *ok = out + (size_t)blockIdx.y * nout;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t chunk = (size_t)CSCI_NT * G, nch = (nout + chunk - 1) / chunk, per = (nch + 7) / 8;
+    const size_t c_end = min(nch, (size_t)(blockIdx.x % 8 + 1) * per);
+    uint32_t mn[G], mf[G], mp[G];
+    auto meta = [&](size_t cj) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const size_t tb = cj * chunk + (size_t)g * CSCI_NT + (size_t)wave * 64;
+            const size_t t = tb + lane;
+            const bool live = tb < nout;
+            mn[g] = t < nout ? (uint32_t)e.c_cnt[t] : 0u;
+            mf[g] = live ? e.c_toff[tb >> 6] : 0u;
+            mp[g] = live ? e.c_pb[tb >> 6] : 0u;
+        }
+    };
+    size_t ci0 = (size_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (ci0 < c_end) meta(ci0);
+    for (size_t ci = ci0; ci < c_end; ci += gridDim.x / 8) {
+        uint32_t a[G], n[G], pb[G], fst[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) { n[g] = mn[g]; fst[g] = mf[g]; pb[g] = mp[g]; }
+        if (ci + gridDim.x / 8 < c_end) meta(ci + gridDim.x / 8);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint32_t first = fst[g];
+            uint32_t x = n[g];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+                if (lane >= o) x += y;
+            }
+            a[g] = first + x - n[g];
+        }
+        uint32_t dp[G][CSCI_U];
+        float fr[G][CSCI_U];
+        TF wv[G][CSCI_U];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int j = 0; j < CSCI_U; ++j) {
+                dp[g][j] = (uint32_t)j < n[g] ? (uint32_t)e.c_dp[a[g] + j] : 0u;
+                fr[g][j] = (uint32_t)j < n[g] ? e.c_fr[a[g] + j] : 0.0f;
+            }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int j = 0; j < CSCI_U; ++j) wv[g][j] = (uint32_t)j < n[g] ? vk[pb[g] + dp[g][j]] : (TF)0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const size_t t = ci * chunk + (size_t)g * CSCI_NT + threadIdx.x;
+            TF acc = (TF)0;
+#pragma unroll
+            for (int j = 0; j < CSCI_U; ++j)
+                if ((uint32_t)j < n[g]) acc = fmat((TF)fr[g][j], wv[g][j], acc);
+            for (uint32_t j = CSCI_U; j < n[g]; ++j)
+                acc = fmat((TF)e.c_fr[a[g] + j], vk[pb[g] + (uint32_t)e.c_dp[a[g] + j]], acc);
+            if (t < nout) ok[t] = acc;
+        }
+    }
+}
+template <class TF>
+hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, bool any_c16, bool any_wide,
+                     hipStream_t st) {
     if (nexp <= 0) return hipSuccess;
     const size_t chunk = (size_t)CSCI_NT * csci_g<TF>();
     const unsigned gx = (unsigned)((std::min<size_t>((nout + chunk - 1) / chunk, (size_t)std::max(1, 16384 / nexp)) + 7) / 8 * 8);
-    hipLaunchKernelGGL((k_csc_image<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
+    if (any_c16) hipLaunchKernelGGL((k_csc_image16<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
+    if (any_wide)
+        hipLaunchKernelGGL((k_csc_image<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out, (int)any_c16);
     return hipGetLastError();
 }
-template hipError_t csc_image<float>(const ExpDev *, int, const float *, size_t, float *, hipStream_t);
-template hipError_t csc_image<double>(const ExpDev *, int, const double *, size_t, double *, hipStream_t);
+template hipError_t csc_image<float>(const ExpDev *, int, const float *, size_t, float *, bool, bool, hipStream_t);
+template hipError_t csc_image<double>(const ExpDev *, int, const double *, size_t, double *, bool, bool,
+                                      hipStream_t);
 
 template hipError_t csc_apply<float>(const ExpDev *, const float *, size_t, float *, hipStream_t);
 template hipError_t csc_apply<double>(const ExpDev *, const double *, size_t, double *, hipStream_t);

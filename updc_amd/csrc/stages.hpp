@@ -39,6 +39,12 @@ hipError_t csc_count(const ExpDev *e_dev, uint32_t nin, const TW *w, uint32_t *t
 template <class TW>
 hipError_t csc_fill(const ExpDev *e_dev, uint32_t nin, const TW *w, uint32_t *cursor, uint2 *tent, hipStream_t st);
 hipError_t csc_sort(const uint32_t *toff, uint2 *tent, size_t nout, hipStream_t st);
+// Compact copy of a sorted CSC (ExpDev::c_*): c_toff[ntiles+1], c_pb[ntiles],
+// c_cnt[nout], c_dp/c_fr[ncsc], ntiles = ceil(nout/64).  flags (device, zeroed
+// by the caller): bit 0 a target with more than 255 entries, bit 1 a tile
+// spanning more than 65535 source pixels (then the copy is unusable).
+hipError_t csc16_build(const uint32_t *toff, const uint2 *tent, size_t nout, uint32_t *c_toff, uint32_t *c_pb,
+                       uint8_t *c_cnt, uint16_t *c_dp, float *c_fr, unsigned int *flags, hipStream_t st);
 
 // ---- weights (updc_core.c:364-373) --------------------------------------
 // F32 semantics (TW = float) or FP64 (TW = double).  err bits UPDC_E_WEIGHT /
@@ -89,8 +95,11 @@ hipError_t commit_diag(ExpDiag *cur, int nexp, IterDiag *icur, ExpDiag *hist, It
                        unsigned int *ctr, hipStream_t st);
 // out[t] = (S_k^T v)[t] for one exposure through the CSC gather (H*W targets).
 template <class TF> hipError_t csc_apply(const ExpDev *e_dev, const TF *v, size_t nout, TF *out, hipStream_t st);
-// All exposures of the batch: out[k * nout + t] = (S_k^T v_k)[t] over the CSC.
+// All exposures of the batch: out[k * nout + t] = (S_k^T v_k)[t] over the CSC,
+// the compact copy where an exposure has one (any_c16: some do; any_wide:
+// some do not).
 template <class TF>
-hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, hipStream_t st);
+hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, bool any_c16, bool any_wide,
+                     hipStream_t st);
 
 }  // namespace updc

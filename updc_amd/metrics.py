@@ -26,7 +26,7 @@ def stats(ref: np.ndarray, new: np.ndarray) -> dict:
 
 def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[int], ncsc: list[int],
                 lm_bytes: list[int], e: int, c: int, e_d: int, e_a: int | None = None,
-                fused_inverse: bool = False) -> dict:
+                fused_inverse: bool = False, adj_csc_bytes: list[int] | None = None) -> dict:
     """Algorithmic HBM bytes per launch of each iteration kernel: every
     operand read once and every result written once (FFT-internal LDS passes
     and re-reads excluded).  e/c = real/complex element bytes of the FFT
@@ -34,7 +34,9 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
     a summed adjoint spectrum wider than the FFT type (the mixed tier), which
     leaves the column pass whole (FH rows) and is inverted in that type by
     col_inv_corr and row_c2r_corr; with fused_inverse the column inverse runs
-    inside col_adj_accum, which then writes the H inverted rows."""
+    inside col_adj_accum, which then writes the H inverted rows.
+    adj_csc_bytes: the transposed maps the adjoint-image gather reads (the
+    compact copies where the exposures have them); default the 8-byte form."""
     N = len(nin)
     NC = FW // 2 + 1
     nout = W * H
@@ -53,7 +55,7 @@ def stage_bytes(W: int, H: int, FW: int, FH: int, nin: list[int], nlinks: list[i
         # CSC + wr + spectrum rows
         "row_r2c_gather": csc + pix * e + spec_k,
         # split form: CSC + wr in, N adjoint images out; then those images in, spectra out
-        "adjoint_image": csc + pix * e + N * nout * e,
+        "adjoint_image": (csc if adj_csc_bytes is None else sum(adj_csc_bytes)) + pix * e + N * nout * e,
         "row_r2c_adj": N * nout * e + spec_k,
         "col_adj_accum": spec_k + psf + spec_x,
         "row_c2r_corr": spec_x + nout * e,

@@ -249,7 +249,10 @@ class Context:
         check(self.lib.updc_get_exposure_info(self.h, k, C.byref(nin), C.byref(nl), C.byref(enc), C.byref(ncsc)))
         n = C.c_size_t()
         check(self.lib.updc_linkmap_size(self.h, k, C.byref(n)))
-        return dict(nin=nin.value, nlinks=nl.value, encoding=enc.value, ncsc=ncsc.value, linkmap_bytes=n.value)
+        compact, cb = C.c_int(), C.c_uint64()
+        check(self.lib.updc_get_csc_layout(self.h, k, C.byref(compact), C.byref(cb)))
+        return dict(nin=nin.value, nlinks=nl.value, encoding=enc.value, ncsc=ncsc.value, linkmap_bytes=n.value,
+                    csc_compact=bool(compact.value), csc_bytes=cb.value)
 
     def reset_frozen(self) -> None:
         check(self.lib.updc_reset_frozen(self.h))
