@@ -115,6 +115,33 @@ typedef struct {
     double ms_total;
 } updc_iter_info;
 
+/* Set-up stage times of a context in seconds (the setup rows of the
+ * reference's timing CSV, updc_core.c:1487-1495): the FFT shape autotune and
+ * the transform tables (fft_autotune, fft_plan), the PSF spectra including
+ * their cache (psf_cache), LinkMap build or validation and upload over all
+ * updc_add_exposure* calls (polygon_links), weight scale and normalisation
+ * (weight_scale), the transposed LinkMaps (no reference counterpart: the
+ * reference scatters), the initial drizzle and the denominator
+ * (initial_drizzle, sensitivity). */
+typedef struct {
+    double fft_autotune, fft_plan, psf_spectra, linkmaps, weight_scale, csc_transpose, initial_drizzle, sensitivity;
+} updc_setup_times;
+
+/* One iteration's timed stages in seconds, named after the reference's
+ * per-iteration timing rows (updc_core.c:1496-1505).  eager = 1: measured
+ * with HIP events around the launches; the X^ column transform is fused into
+ * the forward stage, the adjoint-image gather counts as detector time, and
+ * with the update applied inside the correction's inverse transform that
+ * time is in adjoint_reduce_ifft and update is 0.  The per-exposure form
+ * times only the fused exposure wall.  eager = 0 (hipGraph replay): only total
+ * is measured (the call's average); the stage fields are NaN. */
+typedef struct {
+    int32_t iteration;
+    int32_t eager;
+    double estimate_fft, fused_exposure_wall, forward_ifft_critical, fused_detector_critical, scatter_fft_critical,
+        adjoint_reduce_ifft, update, total;
+} updc_iter_times;
+
 /* fft_width/fft_height of 0 pick the shape at prepare: the smallest even
  * 7-smooth width and 7-smooth height >= target + max PSF - 1
  * (updc_core.c:117-126), or with UPDC_FLAG_AUTOTUNE the fastest candidate
@@ -211,6 +238,11 @@ int updc_iterate(updc_ctx *ctx, int n, updc_freeze_row *rows, updc_iter_info *in
  * per-iteration CSV and stdout lines, updc_core.c:1410-1479).  Both equal n
  * after a successful call.  Does not reset updc_last_error(). */
 int updc_get_iterate_progress(updc_ctx *ctx, int *completed, int *freeze_rows);
+/* Set-up times of the context (valid after updc_prepare). */
+int updc_get_setup_times(updc_ctx *ctx, updc_setup_times *out);
+/* Per-iteration stage times of the last updc_iterate call: *count entries
+ * (its n), the first min(count, capacity) copied to out. */
+int updc_get_iteration_times(updc_ctx *ctx, updc_iter_times *out, int capacity, int *count);
 
 /* Results and setup products (host copies; f32 outputs are what the reference
  * writes, f64 variants return the working precision). */

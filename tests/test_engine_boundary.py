@@ -112,7 +112,37 @@ def test_dropin_executable_end_to_end(small_ds, tmp_path):
     assert sorted(res.iterations) == [1, 2]
     assert len(res.freeze_rows) == 2 * len(ds.geoms)
     assert np.isfinite(res.iterations[2]).all() and np.isfinite(res.wht).all()
-    assert "total_solver" in res.timing
+    # the reference's timing rows (updc_core.c:1487-1508), every value seconds
+    setup = ["fft_autotune", "fft_plan", "psf_cache", "polygon_links", "persistent_mmap", "mmap_prewarm",
+             "data_cache", "weight_scale", "initial_drizzle", "sensitivity"]
+    per_it = ["estimate_fft", "fused_exposure_wall", "forward_ifft_critical", "fused_detector_critical",
+              "scatter_fft_critical", "adjoint_reduce_ifft", "update", "output", "total"]
+    names = list(res.timing)
+    assert names[0] == "fft_autotune" and names[-1] == "total_solver"
+    assert all(n in res.timing for n in setup)
+    for i in (1, 2):
+        rows_i = [f"iteration_{i:02d}_{n}" for n in per_it]
+        assert [n for n in names if n.startswith(f"iteration_{i:02d}_")] == rows_i
+        # eager launches (the drop-in's default): every stage measured
+        t = {n: res.timing[f"iteration_{i:02d}_{n}"] for n in per_it}
+        assert all(np.isfinite(v) and v >= 0 for v in t.values()), t
+        assert t["fused_exposure_wall"] == pytest.approx(
+            t["forward_ifft_critical"] + t["fused_detector_critical"] + t["scatter_fft_critical"], abs=2e-6)
+        assert t["estimate_fft"] > 0 and t["fused_detector_critical"] > 0 and t["total"] >= t["output"]
+    assert res.timing["weight_scale"] > 0 and res.timing["sensitivity"] > 0 and res.timing["polygon_links"] > 0
+
+    print("case: drop-in, hipGraph replay, stdout lines", flush=True)
+    env_g = E.solver_env(False)
+    env_g["UPDC_HIP_GRAPH"] = "1"
+    argv = _argv(ds, tmp_path, manifest, 2, 0, prefix="graph")
+    out = subprocess.run([str(a) for a in argv], capture_output=True, text=True, timeout=120, env=env_g)
+    assert out.returncode == 0, out.stderr
+    lines = [l for l in out.stdout.splitlines() if l.startswith("UPDC iteration") and "/2:" in l]
+    assert len(lines) == 2 and lines[1].endswith(" SAVE") and not lines[0].endswith(" SAVE")
+    assert "[estimateFFT nan fusedWall nan" in lines[0] and "output " in lines[0] and ", wall " in lines[0]
+    gres = E.run_core(argv, env=env_g, timeout=120)
+    assert np.isnan(gres.timing["iteration_01_estimate_fft"]) and gres.timing["iteration_01_total"] > 0
+    assert np.array_equal(gres.iterations[2], res.iterations[2])
 
     # the same solve in memory (F32 tier, LinkMaps built on the GPU from the vertices)
     c = Context(ds.W, ds.H, "f32", graph=True)

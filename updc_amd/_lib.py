@@ -34,6 +34,18 @@ class FreezeRow(C.Structure):
                 ("n_total_frozen", C.c_uint64), ("min_raw_mu", C.c_double)]
 
 
+class SetupTimes(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("fft_autotune", "fft_plan", "psf_spectra", "linkmaps", "weight_scale",
+                                         "csc_transpose", "initial_drizzle", "sensitivity")]
+
+
+class IterTimes(C.Structure):
+    _fields_ = [("iteration", C.c_int32), ("eager", C.c_int32)] + [
+        (n, C.c_double) for n in ("estimate_fft", "fused_exposure_wall", "forward_ifft_critical",
+                                  "fused_detector_critical", "scatter_fft_critical", "adjoint_reduce_ifft", "update",
+                                  "total")]
+
+
 class IterInfo(C.Structure):
     _fields_ = [("iteration", C.c_int32), ("pad", C.c_int32), ("signed_sum", C.c_double),
                 ("n_negative", C.c_uint64), ("n_new_frozen", C.c_uint64), ("n_total_frozen", C.c_uint64),
@@ -78,6 +90,8 @@ SIGNATURES = {
     "updc_prepare": (C.c_int, [_P]),
     "updc_iterate": (C.c_int, [_P, C.c_int, C.POINTER(FreezeRow), C.POINTER(IterInfo)]),
     "updc_get_iterate_progress": (C.c_int, [_P, _I, _I]),
+    "updc_get_setup_times": (C.c_int, [_P, C.POINTER(SetupTimes)]),
+    "updc_get_iteration_times": (C.c_int, [_P, C.POINTER(IterTimes), C.c_int, _I]),
     "updc_get_estimate": (C.c_int, [_P, _F]),
     "updc_get_estimate_f64": (C.c_int, [_P, _D]),
     "updc_set_estimate_f64": (C.c_int, [_P, _D]),

@@ -171,7 +171,7 @@ int env_int(const char *name, int dflt) {
 void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_t, int iterations,
              const char *prefix, const char *wht_out, int fftw, int expw, int prew, int plan_measure, int autotune,
              int retune, const char *wisdom, int save_every) {
-    (void)fftw; (void)expw; (void)plan_measure; (void)wisdom;
+    (void)fftw; (void)plan_measure; (void)wisdom;
     if (W < 1 || H < 1 || iw < 1 || ih < 1 || n < 1 || iterations < 1 || fftw < 1 || expw < 1 || prew < 1 ||
         save_every < 0)
         throw DErr(UPDC_ERR_ARG, "invalid dimensions/exposures/iterations/workers/save_every");
@@ -182,6 +182,9 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
     const bool persist = pc && std::strcmp(pc, "0") != 0;
     const bool quiet = env_int("UPDC_HIP_QUIET", 0) != 0;
     std::printf("Persistent C preprocessing cache: %s\n", persist ? "enabled" : "disabled");
+    if (expw > 1)
+        std::printf("exposure_workers=%d: the GPU sums the exposures' adjoint spectra in exposure order, not as "
+                    "per-worker partial sums (updc_core.c:1214-1216, 1280-1285)\n", expw);
     std::vector<Row> rows = parse_manifest(manifest, n);
 
     // precision tier of the drop-in: reference float32 by default
@@ -190,7 +193,10 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
         if (!std::strcmp(p, "f64")) prec = UPDC_PREC_F64;
         else if (!std::strcmp(p, "mixed")) prec = UPDC_PREC_MIXED;
     }
-    unsigned flags = env_int("UPDC_HIP_GRAPH", 1) ? UPDC_FLAG_GRAPH : 0u;
+    // eager launches by default, so every per-iteration stage of the timing
+    // CSV is measured; UPDC_HIP_GRAPH=1 replays a hipGraph (per-iteration
+    // totals only, the stage rows then read nan)
+    unsigned flags = env_int("UPDC_HIP_GRAPH", 0) ? UPDC_FLAG_GRAPH : 0u;
     if (quiet) flags |= UPDC_FLAG_QUIET;
     // FFT shape: autotuned on the GPU when asked, with its result cached in
     // the manifest's directory when the persistent cache is on and no retune
@@ -202,7 +208,6 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
     if (autotune && persist) check(updc_ctx_set_fft_cache(g.c, cache_root(manifest).c_str(), retune));
     if (persist) check(updc_ctx_set_setup_cache(g.c, cache_root(manifest).c_str()));
 
-    double t0 = now();
     double t_links = 0, t_data = 0;
     for (int k = 0; k < n; ++k) {
         const Row &r = rows[k];
@@ -242,10 +247,7 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
                     (unsigned long long)nl, enc == UPDC_LINKMAP_DELTA16 ? "DELTA16" : "ABS32");
         t_links += now() - tl;
     }
-    const double t_setup_in = now() - t0;
-    t0 = now();
     check(updc_prepare(g.c));
-    const double t_prepare = now() - t0;
     double ws = 0, ms = 0;
     int fw = 0, fh = 0;
     uint64_t masked = 0;
@@ -262,6 +264,7 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
     std::fprintf(fcsv, "iteration,exposure,n_new_frozen,n_total_frozen,min_raw_mu,mu_marker,frozen_ratio\n");
     std::vector<float> est(nout);
     std::vector<double> iter_s(iterations, 0.0), out_s(iterations, 0.0);
+    std::vector<updc_iter_times> stage(iterations);
     int done = 0;
     try {
         while (done < iterations) {
@@ -274,32 +277,44 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
             const double tc = now();
             const int rc = updc_iterate(g.c, chunk, fr.data(), info.data());
             const double el = now() - tc;
+            const std::string failure = rc != UPDC_OK ? updc_last_error() : "";
             // after a numeric failure only the iterations the reference would
             // have finished print and log (updc_core.c:1425-1431, 1473-1478)
             int completed = chunk, with_rows = chunk;
             if (rc != UPDC_OK) check(updc_get_iterate_progress(g.c, &completed, &with_rows));
-            for (int i = 0; i < with_rows; ++i) {
+            for (int i = 0; i < with_rows; ++i)
                 for (int k = 0; k < n; ++k) {
                     const updc_freeze_row &r = fr[(size_t)i * n + k];
                     std::fprintf(fcsv, "%d,%d,%llu,%llu,%.9g,%.9g,1\n", done + i + 1, k + 1,
                                  (unsigned long long)r.n_new_frozen, (unsigned long long)r.n_total_frozen,
                                  (double)(float)r.min_raw_mu, (double)1.0e-12f);
                 }
-                if (i >= completed) break;
-                iter_s[done + i] = el / chunk;
-                if (!quiet)
-                    std::printf("UPDC iteration %d/%d: signed sum %.9g, negative %llu/%zu, frozen +%llu total %llu, "
-                                "min raw mu %.9g, GPU %.3f ms\n",
-                                done + i + 1, iterations, info[i].signed_sum, (unsigned long long)info[i].n_negative,
-                                nout, (unsigned long long)info[i].n_new_frozen,
-                                (unsigned long long)info[i].n_total_frozen, info[i].min_raw_mu, info[i].ms_total);
+            int cnt = 0;
+            const int trc = updc_get_iteration_times(g.c, stage.data() + done, chunk, &cnt);
+            for (int i = 0; i < completed; ++i) iter_s[done + i] = el / chunk;
+            auto line = [&](int i, bool saved) {
+                if (quiet) return;
+                const updc_iter_times &t = stage[done + i];
+                std::printf("UPDC iteration %d/%d: signed sum %.9g, negative %llu/%zu, frozen +%llu total %llu, "
+                            "min raw mu %.9g, wall %.3f s [estimateFFT %.3f fusedWall %.3f forwardIFFTcrit %.3f "
+                            "detectorCrit %.3f scatterFFTcrit %.3f adjointReduceIFFT %.3f update %.3f output %.3f]%s\n",
+                            done + i + 1, iterations, info[i].signed_sum, (unsigned long long)info[i].n_negative,
+                            nout, (unsigned long long)info[i].n_new_frozen,
+                            (unsigned long long)info[i].n_total_frozen, (double)(float)info[i].min_raw_mu,
+                            iter_s[done + i] + out_s[done + i], t.estimate_fft, t.fused_exposure_wall,
+                            t.forward_ifft_critical, t.fused_detector_critical, t.scatter_fft_critical,
+                            t.adjoint_reduce_ifft, t.update, out_s[done + i], saved ? " SAVE" : "");
+            };
+            if (rc != UPDC_OK) {
+                for (int i = 0; i < completed; ++i) line(i, false);
+                throw DErr(rc, failure);
             }
-            check(rc);
-            done = next;
+            check(trc);
+            for (int i = 0; i + 1 < chunk; ++i) line(i, false);
             const double to = now();
             check(updc_get_estimate(g.c, est.data()));
             char path[4096];
-            if (std::snprintf(path, sizeof path, "%s_%02d.bin", prefix, done) >= (int)sizeof path)
+            if (std::snprintf(path, sizeof path, "%s_%02d.bin", prefix, next) >= (int)sizeof path)
                 throw DErr(UPDC_ERR_ARG, "iteration output path is too long");
             FILE *f = std::fopen(path, "wb");
             if (!f) throw DErr(UPDC_ERR_IO, "cannot create iteration output");
@@ -308,8 +323,11 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
                 throw DErr(UPDC_ERR_IO, "iteration output write failed");
             }
             if (std::fclose(f)) throw DErr(UPDC_ERR_IO, "iteration output close failed");
-            std::printf("UPDC iteration %d raw output: %s\n", done, path);
-            out_s[done - 1] = now() - to;
+            std::printf("UPDC iteration %d raw output: %s\n", next, path);
+            out_s[next - 1] = now() - to;
+            line(chunk - 1, true);
+            std::fflush(stdout);
+            done = next;
         }
     } catch (...) {
         std::fclose(fcsv);
@@ -328,18 +346,31 @@ void run_all(const char *manifest, int W, int H, int iw, int ih, int n, int psf_
         }
         if (std::fclose(f)) throw DErr(UPDC_ERR_IO, "WHT output close failed");
     }
-    // timing CSV (updc_core.c:1487-1508), GPU stage names
+    // timing CSV with the reference's rows (updc_core.c:1487-1508): set-up
+    // stages, then nine rows per iteration; every value in seconds.  No
+    // memory-mapped files here (LinkMaps live in HBM): persistent_mmap is the
+    // time spent reading and writing the LinkMap cache files, mmap_prewarm 0.
+    // csc_transpose (the transposed LinkMaps) has no reference counterpart.
+    updc_setup_times su{};
+    check(updc_get_setup_times(g.c, &su));
     const std::string tpath = std::string(prefix) + "_timing.csv";
     FILE *tf = std::fopen(tpath.c_str(), "w");
     if (!tf) throw DErr(UPDC_ERR_IO, "cannot write timing CSV");
-    std::fprintf(tf, "stage,wall_seconds\ninputs_and_linkmaps,%.6f\npolygon_links,%.6f\ndata_cache,%.6f\nprepare,%.6f\n",
-                 t_setup_in, t_links, t_data, t_prepare);
-    char names[16][32];
-    double sms[16];
-    int sn[16], cnt = 16;
-    if (updc_get_stage_times(g.c, names, sms, sn, &cnt) == UPDC_OK)
-        for (int i = 0; i < cnt; ++i) std::fprintf(tf, "last_call_avg_%s_ms,%.6f\n", names[i], sms[i]);
+    std::fprintf(tf,
+                 "stage,wall_seconds\nfft_autotune,%.6f\nfft_plan,%.6f\npsf_cache,%.6f\npolygon_links,%.6f\n"
+                 "persistent_mmap,%.6f\nmmap_prewarm,%.6f\ndata_cache,%.6f\nweight_scale,%.6f\n"
+                 "initial_drizzle,%.6f\nsensitivity,%.6f\ncsc_transpose,%.6f\n",
+                 su.fft_autotune, su.fft_plan, su.psf_spectra, su.linkmaps, std::max(0.0, t_links - su.linkmaps), 0.0,
+                 t_data, su.weight_scale, su.initial_drizzle, su.sensitivity, su.csc_transpose);
     for (int i = 0; i < iterations; ++i) {
+        const updc_iter_times &t = stage[i];
+        std::fprintf(tf, "iteration_%02d_estimate_fft,%.6f\n", i + 1, t.estimate_fft);
+        std::fprintf(tf, "iteration_%02d_fused_exposure_wall,%.6f\n", i + 1, t.fused_exposure_wall);
+        std::fprintf(tf, "iteration_%02d_forward_ifft_critical,%.6f\n", i + 1, t.forward_ifft_critical);
+        std::fprintf(tf, "iteration_%02d_fused_detector_critical,%.6f\n", i + 1, t.fused_detector_critical);
+        std::fprintf(tf, "iteration_%02d_scatter_fft_critical,%.6f\n", i + 1, t.scatter_fft_critical);
+        std::fprintf(tf, "iteration_%02d_adjoint_reduce_ifft,%.6f\n", i + 1, t.adjoint_reduce_ifft);
+        std::fprintf(tf, "iteration_%02d_update,%.6f\n", i + 1, t.update);
         std::fprintf(tf, "iteration_%02d_output,%.6f\n", i + 1, out_s[i]);
         std::fprintf(tf, "iteration_%02d_total,%.6f\n", i + 1, iter_s[i] + out_s[i]);
     }

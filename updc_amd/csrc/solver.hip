@@ -12,6 +12,7 @@
 #include "stages.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <unistd.h>
 #include <cstdio>
@@ -121,6 +122,9 @@ static float overlap(const P q[4], int x, int y) {
 }  // namespace geo
 
 // ------------------------------------------------------------------ helpers
+static double host_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 static bool seven_smooth(long n) {
     if (n < 1) return false;
     for (int p : {2, 3, 5, 7}) while (n % p == 0) n /= p;
@@ -310,6 +314,8 @@ struct Ctx {
     bool prepared = false;
     int iter_done = 0;  // the last updc_iterate: iterations completed (estimate updated)
     int iter_rows = 0;  // ... and iterations whose freeze rows are valid
+    updc_setup_times setup{};                 // set-up stage times (seconds)
+    std::vector<updc_iter_times> iter_times;  // per iteration of the last updc_iterate
     float weight_scale = 0;
     unsigned long long masked = 0;
     std::unique_ptr<PipeBase> pipe;
@@ -520,6 +526,14 @@ template <class TF, class TA> struct Pipe : PipeBase {
     void prepare(Ctx &c) override {
         const int n = N(c);
         const size_t nout = c.nout(), npix = c.npix();
+        double tm = host_seconds();
+        // seconds since the last mark, the stream drained first
+        auto lap = [&]() {
+            HIPCHK(hipStreamSynchronize(c.st));
+            const double t1 = host_seconds(), d = t1 - tm;
+            tm = t1;
+            return d;
+        };
         fft.allow_edge = c.col_edge != 0;
         std::string e = fft.init(c.FW, c.FH);
         if (!e.empty()) throw Error(UPDC_ERR_ARG, e);
@@ -529,6 +543,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             e = fa.init(c.FW, c.FH);
             if (!e.empty()) throw Error(UPDC_ERR_ARG, e);
         }
+        c.setup.fft_plan = lap();
         const FftShape &s = fft.s;
         max_nin = 0;
         for (auto &x : c.ex) max_nin = std::max<uint32_t>(max_nin, (uint32_t)x->nin);
@@ -576,9 +591,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(hipMemcpy(&c.masked, masked.p, 8, hipMemcpyDeviceToHost));
         if (ebits & UPDC_E_WEIGHT) throw Error(UPDC_ERR_NUMERIC, "invalid normalized WHT");
         if (ebits & UPDC_E_YEFF) throw Error(UPDC_ERR_NUMERIC, "negative/non-finite y_eff reached solver");
+        c.setup.weight_scale = lap();
 
         build_csc(c);
         upload_ex(c);
+        c.setup.csc_transpose = lap();
 
         // PSF spectra (cached for the F32 tier in the reference's FFTW layout:
         // FH x (FW/2+1) complex64, row-major)
@@ -632,6 +649,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             }
         }
 
+        c.setup.psf_spectra = lap();
         // initial drizzle (updc_core.c:905-983)
         x.alloc(nout);
         cov.alloc(nout);
@@ -660,6 +678,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         }
         x_init.alloc(nout);
         HIPCHK(hipMemcpyAsync(x_init.p, x.p, nout * sizeof(TF), hipMemcpyDeviceToDevice, c.st));
+        c.setup.initial_drizzle = lap();
 
         // denominator
         D.alloc(nout);
@@ -692,6 +711,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(hipStreamSynchronize(c.st));
         max_sens = std::max(0.0, dkey_inv(kk));
         if (!(max_sens > 0) || !std::isfinite(max_sens)) throw Error(UPDC_ERR_NUMERIC, "invalid H^T W sensitivity");
+        c.setup.sensitivity = lap();
 
         // iteration buffers
         blur.alloc((size_t)n * nout);
@@ -893,14 +913,59 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const int nst = (int)stage_names.size();
         stage_ms.assign(nst + 1, 0.0);
         stage_n.assign(nst + 1, 0);
+        // per iteration, the launches grouped as the reference's timed stages
+        // (updc_core.c:1411-1478): seconds, NaN where graph replay hides them
+        c.iter_times.assign(niter, updc_iter_times{});
+        const double nan = std::nan("");
+        for (int it = 0; it < niter; ++it) {
+            updc_iter_times &r = c.iter_times[it];
+            r.iteration = it + 1;
+            r.eager = use_graph ? 0 : 1;
+            r.estimate_fft = r.fused_exposure_wall = r.forward_ifft_critical = r.fused_detector_critical =
+                r.scatter_fft_critical = r.adjoint_reduce_ifft = r.update = nan;
+            r.total = total_ms * 1e-3 / std::max(1, niter);
+        }
         if (!use_graph) {
-            for (int it = 0; it < niter; ++it)
+            for (int it = 0; it < niter; ++it) {
+                double sec[16] = {0};
                 for (int s = 0; s < nst; ++s) {
                     float ms = 0;
                     HIPCHK(hipEventElapsedTime(&ms, ev[(size_t)it * nev + s], ev[(size_t)it * nev + s + 1]));
                     stage_ms[s] += ms;
                     stage_n[s] += 1;
+                    sec[s] = ms * 1e-3;
                 }
+                updc_iter_times &r = c.iter_times[it];
+                auto sum = [&](std::initializer_list<const char *> names) {
+                    double t = 0;
+                    for (int s = 0; s < nst; ++s)
+                        for (const char *nm : names)
+                            if (stage_names[s] == nm) t += sec[s];
+                    return t;
+                };
+                r.total = 0;
+                for (int s = 0; s < nst; ++s) r.total += sec[s];
+                if (plain) {
+                    // one launch sequence per exposure: blur, detector and
+                    // adjoint transform interleave, only their sum is timed
+                    r.estimate_fft = sum({"plain_xhat"});
+                    r.fused_exposure_wall = sum({"plain_exposures"});
+                    r.adjoint_reduce_ifft = sum({"plain_adjoint_inverse"});
+                    r.update = sum({"update"});
+                } else {
+                    // the X^ column pass runs inside the broadcast column pass
+                    // (forward); the gathers are the detector loop; the adjoint
+                    // row transforms and the summed column pass the scatter
+                    // transforms; the correction's inverse may carry the update
+                    r.estimate_fft = sum({"row_r2c_x"});
+                    r.forward_ifft_critical = sum({"col_fwd_bcast", "row_c2r_blur"});
+                    r.fused_detector_critical = sum({"gather_freeze", "adjoint_image"});
+                    r.scatter_fft_critical = sum({"row_r2c_adj", "row_r2c_gather", "col_adj_accum"});
+                    r.adjoint_reduce_ifft = sum({"col_inv_corr", "row_c2r_corr", "row_c2r_update"});
+                    r.update = sum({"update"});
+                    r.fused_exposure_wall = r.forward_ifft_critical + r.fused_detector_critical + r.scatter_fft_critical;
+                }
+            }
             for (int s = 0; s < nst; ++s) stage_ms[s] /= std::max(1, stage_n[s]);
         }
         stage_ms[nst] = total_ms / std::max(1, niter);
@@ -1799,8 +1864,10 @@ int updc_add_exposure(updc_ctx *ctx, const float *sci, const float *wht, int iw,
         HIPCHK(hipSetDevice(ctx->device));
         check_dims(*ctx, iw, ih, pw, ph);
         HostExp &e = new_exposure(*ctx, sci, wht, iw, ih, psf, pw, ph, psf_t, bg, ped);
+        const double t0 = host_seconds();
         try {
             add_linkmap_image(*ctx, e, linkmap, linkmap_bytes);
+            ctx->setup.linkmaps += host_seconds() - t0;
         } catch (...) {
             ctx->ex.pop_back();
             throw;
@@ -1816,8 +1883,10 @@ int updc_add_exposure_vertices(updc_ctx *ctx, const float *sci, const float *wht
         HIPCHK(hipSetDevice(ctx->device));
         check_dims(*ctx, iw, ih, pw, ph);
         HostExp &e = new_exposure(*ctx, sci, wht, iw, ih, psf, pw, ph, psf_t, bg, ped);
+        const double t0 = host_seconds();
         try {
             build_linkmap(*ctx, e, vert);
+            ctx->setup.linkmaps += host_seconds() - t0;
         } catch (...) {
             ctx->ex.pop_back();
             throw;
@@ -1852,7 +1921,9 @@ int updc_prepare(updc_ctx *ctx) {
             pad_h = std::max(pad_h, e->psf_t ? e->pw : e->ph);
         }
         const int min_w = ctx->W + pad_w - 1, min_h = ctx->H + pad_h - 1;
+        const double ta = host_seconds();
         if (ctx->FW == 0 && ctx->FH == 0 && (ctx->flags & UPDC_FLAG_AUTOTUNE)) autotune_shape(*ctx, min_w, min_h);
+        ctx->setup.fft_autotune = host_seconds() - ta;
         if (ctx->FW == 0) ctx->FW = next_even_smooth(min_w);
         if (ctx->FH == 0) ctx->FH = next_smooth(min_h);
         need(ctx->FW >= min_w && ctx->FH >= min_h, UPDC_ERR_ARG, "FFT shape smaller than target + PSF - 1");
@@ -1891,6 +1962,20 @@ int updc_iterate(updc_ctx *ctx, int n, updc_freeze_row *rows, updc_iter_info *in
     });
 }
 // Leaves updc_last_error() as it is: callers read it after a failed iterate.
+int updc_get_setup_times(updc_ctx *ctx, updc_setup_times *out) {
+    return guarded([&] {
+        need(ctx && out, UPDC_ERR_ARG, "null argument");
+        *out = ctx->setup;
+    });
+}
+int updc_get_iteration_times(updc_ctx *ctx, updc_iter_times *out, int capacity, int *count) {
+    return guarded([&] {
+        need(ctx && count && (out || capacity == 0), UPDC_ERR_ARG, "null argument");
+        const int n = (int)ctx->iter_times.size();
+        for (int i = 0; i < std::min(n, capacity); ++i) out[i] = ctx->iter_times[i];
+        *count = n;
+    });
+}
 int updc_get_iterate_progress(updc_ctx *ctx, int *completed, int *freeze_rows) {
     if (!ctx) return UPDC_ERR_ARG;
     if (completed) *completed = ctx->iter_done;

@@ -321,6 +321,21 @@ class Context:
         check(self.lib.updc_fft_c2r_f64(self.h, _pd(sp), _pd(out)))
         return out
 
+    def setup_times(self) -> dict:
+        """Seconds per set-up stage (updc_get_setup_times)."""
+        t = _lib.SetupTimes()
+        check(self.lib.updc_get_setup_times(self.h, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in t._fields_}
+
+    def iteration_times(self) -> list[dict]:
+        """Per-iteration stage seconds of the last iterate call, named after
+        the reference's timing rows (NaN where graph replay hides a stage)."""
+        n = C.c_int()
+        check(self.lib.updc_get_iteration_times(self.h, None, 0, C.byref(n)))
+        buf = (_lib.IterTimes * max(1, n.value))()
+        check(self.lib.updc_get_iteration_times(self.h, buf, n.value, C.byref(n)))
+        return [{f: getattr(b, f) for f, _ in b._fields_} for b in buf[:n.value]]
+
     def stage_times(self) -> dict:
         cap = 16
         names = (C.c_char * 32 * cap)()
