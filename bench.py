@@ -45,11 +45,14 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_dataset(config: int):
+def make_dataset(config: int, stress: bool = False):
     from updc_amd import synth
     from updc_amd.solver import Context, Exposure
     t0 = time.time()
-    ds = {1: synth.config1, 2: synth.config2, 3: synth.config3, 4: synth.config4, 5: synth.config5}[config]()
+    if stress:
+        ds = synth.config3_stress()
+    else:
+        ds = {1: synth.config1, 2: synth.config2, 3: synth.config3, 4: synth.config4, 5: synth.config5}[config]()
     log(f"synthetic {ds.name}: truth/PSFs/geometry in {time.time() - t0:.1f}s")
     t0 = time.time()
     n = len(ds.geoms)
@@ -149,30 +152,51 @@ def run_stress(args, ds, metrics) -> dict:
         errs[str(it)] = {k: st[k] for k in ("norm_linf", "rel_rms", "max_rel_masked", "max_abs", "rms")}
         log(f"iteration {it}: FP32/batched vs FP64 {errs[str(it)]}")
 
-    # (b) FreezeMu forcing
+    # (b) FreezeMu forcing by the reference's recipe (every background -100),
+    # and (c) the zero-read-noise stress (synth.config3_stress): each tier
+    # against FP64 after 2, 10, 20 and 40 iterations
+    def tiers_vs_fp64(d, label):
+        snaps, rows = {}, {}
+        for prec in ("f64", "mixed", "f32"):
+            c = solve_ctx(d, prec, graph=True)
+            snaps[prec], rows[prec], done = {}, [], 0
+            try:
+                for n in (2, 8, 10, 20):
+                    r, _ = c.iterate(n)
+                    rows[prec] += [(int(q[0]) + done, int(q[1]), int(q[2]), int(q[3])) for q in r]
+                    done += n
+                    snaps[prec][done] = c.estimate(f64=True)
+            except UpdcError as e:
+                log(f"{label} {prec}: {e}")
+            c.close()
+        out = {}
+        for prec in ("mixed", "f32"):
+            out[prec] = {}
+            for it in sorted(snaps[prec]):
+                if it not in snaps["f64"]:
+                    continue
+                st = metrics.stats(snaps["f64"][it], snaps[prec][it])
+                out[prec][str(it)] = {k: st[k] for k in ("norm_linf", "rel_rms", "max_rel_masked")}
+            out[prec]["freeze_rows_equal_to_fp64"] = rows[prec] == rows["f64"][:len(rows[prec])]
+        out["frozen_total_fp64"] = {str(it): sum(r[3] for r in rows["f64"] if r[0] == it) for it in (2, 10, 20, 40)}
+        log(f"{label}: {out}")
+        return out
+
+    saved_bg = list(ds.background)
     ds.background = [-100.0] * nexp
-    forced = {}
-    for name, prec, graph, plain in (("fp32_batched", "mixed", True, False), ("fp64_plain", "f64", False, True)):
-        c = solve_ctx(ds, prec, graph=graph, plain=plain)
-        try:
-            rows, _ = c.iterate(2)
-            forced[name] = dict(rows=[(int(r[0]), int(r[1]), int(r[2]), int(r[3])) for r in rows],
-                                x=c.estimate(f64=True))
-        except UpdcError as e:
-            forced[name] = dict(error=str(e))
-        c.close()
-    fz = {}
-    if all("rows" in v for v in forced.values()):
-        ra, rb = forced["fp32_batched"]["rows"], forced["fp64_plain"]["rows"]
-        inv = all(ra[k][3] == ra[k][2] and ra[nexp + k][3] == ra[k][3] + ra[nexp + k][2] for k in range(nexp))
-        st = metrics.stats(forced["fp64_plain"]["x"], forced["fp32_batched"]["x"])
-        fz = {"frozen_per_exposure_fp32_batched": [r[3] for r in ra[nexp:]],
-              "frozen_per_exposure_fp64": [r[3] for r in rb[nexp:]],
-              "freeze_rows_equal": ra == rb, "invariants_hold": inv,
-              "rel_err_vs_fp64_after_2": {k: st[k] for k in ("norm_linf", "rel_rms")}}
-    else:
-        fz = {k: v.get("error", "ok") for k, v in forced.items()}
-    log(f"FreezeMu forcing (background -100, 2 iterations): {fz}")
+    fz = tiers_vs_fp64(ds, "FreezeMu forcing (background -100)")
+    ds.background = saved_bg
+    sd = make_dataset(3, stress=True)
+    x0 = None
+    with solve_ctx(sd, "f64", graph=False) as c:
+        x0, _ = c.initial()
+        pred = c.model(x0)
+    raw = np.concatenate([p.ravel() + sd.background[k] + sd.pedestal[k] for k, p in enumerate(pred)])
+    act = np.concatenate([w.ravel() for w in sd.wht]) > 0
+    frac_small = float(np.mean(np.abs(raw[act]) < 1.0))
+    stress = tiers_vs_fp64(sd, "zero-read-noise stress")
+    stress["raw_mu_at_x0_below_1_fraction"] = frac_small
+    stress["pedestal"] = sd.pedestal[0]
     return {
         "metric": METRIC, "value": round(its32, 3), "unit": "iters/s", "n_gpus": 1, "steps": K,
         "warmup": args.warmup, "ms_per_step": round(dt32 * 1e3 / K, 4), "higher_is_better": True,
@@ -183,7 +207,8 @@ def run_stress(args, ds, metrics) -> dict:
                    "fp64_plain_iters_per_s": round(its64, 3), "fp32_over_fp64_plain": round(its32 / its64, 3),
                    "rel_err_vs_fp64_at": errs,
                    "freeze_row_mismatches": sum(1 for a, b in zip(rows32, rows64) if a[2:4] != b[2:4]),
-                   "freezemu_forcing": fz},
+                   "freezemu_forcing_background_minus_100": fz,
+                   "zero_read_noise_stress": stress},
         "roofline": None, "cpu_baseline": None,
     }
 

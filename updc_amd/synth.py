@@ -186,6 +186,19 @@ def config3() -> Dataset:
     return ds
 
 
+def config3_stress() -> Dataset:
+    """FreezeMu stress on the config-3 workload: sky 0.01 and no read noise.
+    A detector pixel then sees 0.04 sky counts, so about 96% of sky pixels
+    record 0, the automatic pedestal is 1e-12 (_engine.py:368-372 with sigma
+    = 0), and the drizzled estimate is 0 wherever every overlapping pixel
+    recorded 0: there raw_mu = H_k x + b_k + ped_k starts at the pedestal,
+    and whether it stays above 0 is decided by transform rounding."""
+    ds = config3()
+    ds.read_noise = 0.0
+    ds.name = "config3_stress"
+    return ds
+
+
 def gaussian_mixture_psf(size: int, rng: np.random.Generator, ncomp: int = 3,
                          fwhm_range=(1.8, 3.0)) -> np.ndarray:
     """Normalised sum of `ncomp` Gaussians, FWHM U[fwhm_range], weights
@@ -290,7 +303,8 @@ def observe(ds: Dataset, model: list[np.ndarray], config_seed: int = 0) -> Datas
         lam = np.clip(m, 0, None)
         y = rng.poisson(lam).astype(np.float64) + rng.normal(0.0, ds.read_noise, m.shape)
         var = lam + ds.read_noise ** 2
-        w = (1.0 / var).astype(np.float32)
+        with np.errstate(divide="ignore"):
+            w = (1.0 / var).astype(np.float32)  # inf where var = 0: masked by the weight rule
         if ds.mask_fraction > 0:
             w[rng.uniform(0, 1, m.shape) < ds.mask_fraction] = 0.0
         sci = y.astype(np.float32)
