@@ -405,15 +405,12 @@ def test_wide_rows_take_the_unfused_paths(small_ds):
     p.close()
 
 
-def test_widest_row_fft_and_the_limit_beyond(small_ds):
+def test_widest_one_workgroup_row_fft(small_ds):
     """Rows of M = 4096 points (FFT 8192 wide, radix 8 only) are the widest a
     4-row tile's registers hold (two radix-8 butterflies per thread per
-    stage): every tier matches the FP64 oracle at that shape.  M = 4200
-    (FFT 8400, a radix-7 stage) needs three radix-7 butterflies per thread, and
-    the context refuses the shape with an error naming the row pass instead of
-    running anything."""
+    stage): every tier matches the FP64 oracle at that shape.  Wider rows
+    take the four-step path (test_gpu_four_step.py)."""
     from oracle import oracle as O
-    from updc_amd._lib import UpdcError
     shape = (8192, 112)
     print(f"case: FFT {shape}, widest rows", flush=True)
     ref = O.RefSolver(small_ds.W, small_ds.H, oracle_exposures(small_ds), "f64", fft_shape=shape)
@@ -426,11 +423,6 @@ def test_widest_row_fft_and_the_limit_beyond(small_ds):
         print(f"  {prec}: norm_linf {st['norm_linf']:.3e}", flush=True)
         assert st["norm_linf"] < tol, (prec, st)
         c.close()
-    print("case: FFT (8400, 112) refused", flush=True)
-    c = _ctx(small_ds, "f32", fft=(8400, 112))
-    with pytest.raises(UpdcError, match="FFT width too large"):
-        c.prepare()
-    c.close()
 
 
 def test_radix16_row_plan_matches_oracle(small_ds):

@@ -13,6 +13,7 @@
 #include "fft.hpp"
 #include <cmath>
 #include <algorithm>
+#include <type_traits>
 
 namespace updc {
 
@@ -755,6 +756,184 @@ __global__ void k_spec(const Cx<T> *__restrict__ a, const Cx<T> *__restrict__ ps
     }
 }
 
+// ------------------------------------------------------------------ four-step lines
+// (the decomposition is described with LongPlan in fft.hpp).  Line o of a
+// LineIO is line o % lpb of batch o / lpb.
+template <class T> __device__ __forceinline__ size_t line_base(const LineIO &io, size_t o) {
+    const size_t b = o / (size_t)io.lpb;
+    return b * io.bstride + (o - b * (size_t)io.lpb) * io.pitch;
+}
+// target t of exposure e: S_k^T wr at t, the fmaf chain over the transposed
+// map's entries in ascending source pixel (as the row gather above)
+template <class T> __device__ __forceinline__ T gather_target(const ExpDev &e, const T *__restrict__ w, uint32_t t) {
+    T acc = (T)0;
+    for (uint32_t a = e.toff[t], z = e.toff[t + 1]; a < z; ++a) {
+        const uint2 x = e.tent[a];
+        acc = fmat((T)__uint_as_float(x.y), w[x.x], acc);
+    }
+    return acc;
+}
+template <class T, class TS, int LD>
+__device__ __forceinline__ Cx<T> long_load(const LineIO &io, size_t o, int n) {
+    if (LD == LL_CX) {
+        const Cx<TS> v = n < io.nvalid ? ((const Cx<TS> *)io.p)[line_base<T>(io, o) + n] : cx<TS>((TS)0, (TS)0);
+        return cx<T>((T)v.re, (T)v.im);
+    } else if (LD == LL_CX_MUL) {
+        const size_t b = o / (size_t)io.lpb, l = o - b * (size_t)io.lpb;
+        const Cx<T> x = ((const Cx<T> *)io.p)[l * io.pitch + n];
+        const Cx<T> q = ((const Cx<T> *)io.psf)[b * io.pbstride + l * io.ppitch + n];
+        return cmul(x, q);
+    } else if (LD == LL_REAL) {
+        const T *row = (const T *)io.p + line_base<T>(io, o);
+        const int c = 2 * n;
+        return cx<T>(c < io.nvalid ? row[c] : (T)0, c + 1 < io.nvalid ? row[c + 1] : (T)0);
+    } else {  // LL_GATHER: line = target row r of exposure b, columns c < W
+        const size_t b = o / (size_t)io.lpb, r = o - b * (size_t)io.lpb;
+        const ExpDev &e = io.ex[b];
+        const T *w = (const T *)io.wr + e.pix;
+        const int c = 2 * n, W = io.nvalid;
+        const uint32_t t = (uint32_t)(r * (size_t)W) + (uint32_t)c;
+        return cx<T>(c < W ? gather_target<T>(e, w, t) : (T)0, c + 1 < W ? gather_target<T>(e, w, t + 1) : (T)0);
+    }
+}
+
+// Pass A: workgroup (line o, block of B sub-lines n1b..n1b+B-1).
+template <class T, class TS, bool INV, int LD, int NT, int VMAX>
+__global__ __launch_bounds__(NT) void k_long_a(FftLen p2, const Cx<T> *__restrict__ roots, int L1, int lgb, int nblk,
+                                               size_t line0, LineIO src, Cx<T> *__restrict__ tmp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
+    const size_t o = line0 + blockIdx.x / (unsigned)nblk;
+    const int n1b = (int)(blockIdx.x % (unsigned)nblk) << lgb;
+    const int L2 = p2.L, B = 1 << lgb, nsl = L2 << lgb;
+    for (int idx = threadIdx.x; idx < nsl; idx += NT) {
+        const int j = idx & (B - 1), n2 = idx >> lgb, n1 = n1b + j;
+        buf[idx] = n1 < L1 ? long_load<T, TS, LD>(src, o, n1 + L1 * n2) : cx<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    fft_lds<INV, T, NT, VMAX>(buf, p2, lgb);
+    Cx<T> *d = tmp + o * (size_t)L1 * L2;
+    for (int idx = threadIdx.x; idx < nsl; idx += NT) {
+        const int j = idx & (B - 1), k2 = idx >> lgb, n1 = n1b + j;
+        if (n1 >= L1) continue;
+        Cx<T> w = roots[n1 * k2];  // n1 k2 < L1 L2: no reduction needed
+        if (INV) w.im = -w.im;
+        d[(size_t)k2 * L1 + n1] = cmul(buf[idx], w);
+    }
+}
+
+// Pass B: workgroup (line o, block of B sub-lines k2b..k2b+B-1).
+template <class T, bool INV, int ST, int NT, int VMAX>
+__global__ __launch_bounds__(NT) void k_long_b(FftLen p1, int L2, int lgb, int nblk, size_t line0,
+                                               const Cx<T> *__restrict__ tmp, LineIO dst) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
+    const size_t o = line0 + blockIdx.x / (unsigned)nblk;
+    const int k2b = (int)(blockIdx.x % (unsigned)nblk) << lgb;
+    const int L1 = p1.L, B = 1 << lgb, nsl = L1 << lgb;
+    const Cx<T> *srcl = tmp + o * (size_t)L1 * L2;
+    // sub-line j is the contiguous run srcl[(k2b + j) L1 ...]: lanes walk it in order
+    for (int idx = threadIdx.x; idx < nsl; idx += NT) {
+        const int j = idx / L1, n1 = idx - j * L1, k2 = k2b + j;
+        buf[(n1 << lgb) + j] = k2 < L2 ? srcl[(size_t)k2 * L1 + n1] : cx<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    fft_lds<INV, T, NT, VMAX>(buf, p1, lgb);
+    const size_t base = line_base<T>(dst, o);
+    for (int idx = threadIdx.x; idx < nsl; idx += NT) {
+        const int j = idx & (B - 1), k1 = idx >> lgb, k2 = k2b + j;
+        if (k2 >= L2) continue;
+        const int k = k2 + L2 * k1;
+        const Cx<T> v = buf[idx];
+        if (ST == LS_CX) {
+            if (k < dst.nvalid) ((Cx<T> *)dst.p)[base + k] = v;
+        } else {
+            T *row = (T *)dst.p + base;
+            const T sc = (T)dst.scale;
+            const int c = 2 * k;
+            if (c < dst.nvalid) row[c] = v.re * sc;
+            if (c + 1 < dst.nvalid) row[c + 1] = v.im * sc;
+        }
+    }
+}
+
+// Real split of four-step row transforms: Z (row-major lines b * rows + r of
+// M points) -> column-major half spectra out[b][v RP + r], v in [0, M], the
+// same split as k_row_r2c.  A 16-row x 64-column tile passes through LDS so
+// both sides are read and written in runs.
+template <class T>
+__global__ __launch_bounds__(256) void k_long_split(const Cx<T> *__restrict__ Z, int M, int rows,
+                                                    const Cx<T> *__restrict__ post, Cx<T> *__restrict__ out,
+                                                    size_t out_bstride, int RP) {
+    constexpr int TR = 16, TV = 64;
+    __shared__ Cx<T> tile[TR][TV + 1];
+    const int v0 = blockIdx.x * TV, r0 = blockIdx.y * TR, b = blockIdx.z;
+    const T half = (T)0.5;
+    for (int i = threadIdx.x; i < TR * TV; i += 256) {
+        const int rr = i / TV, vv = i - rr * TV, r = r0 + rr, v = v0 + vv;
+        if (r >= rows || v > M) continue;
+        const Cx<T> *zl = Z + ((size_t)b * rows + r) * M;
+        const Cx<T> zv = zl[v == M ? 0 : v];
+        const Cx<T> zm = cconj(zl[v == 0 ? 0 : M - v]);
+        const Cx<T> fe = cscale(zv + zm, half);
+        const Cx<T> d = zv - zm;
+        const Cx<T> fo = cx<T>(d.im * half, -d.re * half);
+        tile[rr][vv] = fe + cmul(post[v], fo);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TR * TV; i += 256) {
+        const int vv = i / TR, rr = i - vv * TR, r = r0 + rr, v = v0 + vv;
+        if (r < rows && v <= M) out[(size_t)b * out_bstride + (size_t)v * RP + r] = tile[rr][vv];
+    }
+}
+
+// Merge of four-step row inverses: column-major half spectra -> row-major
+// half-length complex lines Z[b * rows + r][v], v < M, the same merge as
+// k_row_c2r.
+template <class T>
+__global__ __launch_bounds__(256) void k_long_merge(const Cx<T> *__restrict__ in, size_t in_bstride, int RP, int M,
+                                                    int rows, const Cx<T> *__restrict__ post, Cx<T> *__restrict__ Z) {
+    constexpr int TR = 16, TV = 64;
+    __shared__ Cx<T> tile[TR][TV + 1];
+    const int v0 = blockIdx.x * TV, r0 = blockIdx.y * TR, b = blockIdx.z;
+    const Cx<T> *y = in + (size_t)b * in_bstride;
+    for (int i = threadIdx.x; i < TR * TV; i += 256) {
+        const int vv = i / TR, rr = i - vv * TR, r = r0 + rr, v = v0 + vv;
+        if (r >= rows || v >= M) continue;
+        const Cx<T> yv = y[(size_t)v * RP + r], yu = y[(size_t)(M - v) * RP + r];
+        const Cx<T> e1 = yv + cconj(yu), o1 = cmul(yv - cconj(yu), cconj(post[v]));
+        tile[rr][vv] = cx<T>(e1.re - o1.im, e1.im + o1.re);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TR * TV; i += 256) {
+        const int rr = i / TV, vv = i - rr * TV, r = r0 + rr, v = v0 + vv;
+        if (r < rows && v < M) Z[((size_t)b * rows + r) * M + v] = tile[rr][vv];
+    }
+}
+
+// Summed adjoint spectrum of four-step columns: out[v opitch + r] =
+// sum_k conj(psf_k) X_k (k ascending, products in T, sum in TA, as k_col's
+// COL_ADJ / COL_ACC), X_k line-major (k NC + v) FH + r.
+template <class T, class TA>
+__global__ __launch_bounds__(256) void k_long_adjsum(const Cx<T> *__restrict__ X, int NC, int FH, int nexp,
+                                                     const Cx<T> *__restrict__ psf, size_t pbs, int prp,
+                                                     Cx<TA> *__restrict__ out, int opitch) {
+    const size_t n = (size_t)NC * FH;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t v = i / FH, r = i - v * FH;
+        Cx<TA> acc = cx<TA>((TA)0, (TA)0);
+        for (int k = 0; k < nexp; ++k) {
+            const Cx<T> p = psf[(size_t)k * pbs + v * prp + r];
+            const Cx<T> s = X[(size_t)k * n + i];
+            const T pre = s.re * p.re + s.im * p.im;
+            const T pim = s.im * p.re - s.re * p.im;
+            acc.re += (TA)pre;
+            acc.im += (TA)pim;
+        }
+        out[v * opitch + r] = acc;
+    }
+}
+
 // ------------------------------------------------------------------ host side
 // Thread counts and per-thread register budgets.  A column workgroup holds
 // one FH-point column (44.8 KB f32 / 89.6 KB f64 at FH = 5600) and runs 8
@@ -948,36 +1127,112 @@ template <class T, class TA> static void *col_acc_e_fn(int re) {
     return nullptr;
 }
 
+// ------------------------------------------------------------------ four-step host side
+// 256 threads, 16 complex values per thread and stage; B = 16 (FP32) or 8
+// (FP64) sub-lines make 128-byte runs.
+template <class T> struct LongCfg;
+template <> struct LongCfg<float> { static constexpr int NT = 256, VMAX = 16, LGB = 4; };
+template <> struct LongCfg<double> { static constexpr int NT = 256, VMAX = 16, LGB = 3; };
+static constexpr size_t LONG_LDS = 64 * 1024;  // per workgroup: several resident per CU
+
+static bool seven_smooth_len(long n) {
+    for (int p : {2, 3, 5, 7})
+        while (n > 1 && n % p == 0) n /= p;
+    return n == 1;
+}
+
+// L = L1 x L2 with both sub-plans fitting B interleaved sub-lines in one
+// workgroup's registers and LDS: the widest B first (the longest global
+// runs), then the most even split.
+template <class T> static std::string long_plan(int L, LongPlan<T> &lp) {
+    constexpr int NT = LongCfg<T>::NT, VMAX = LongCfg<T>::VMAX;
+    int best1 = 0, bestlg = -1;
+    long bestd = 0;
+    for (int L1 = 2; L1 <= L / 2; ++L1) {
+        if (L % L1) continue;
+        const int L2 = L / L1;
+        FftLen a{}, b{};
+        if (!factor_radices(L1, a, false, nullptr, true) || !factor_radices(L2, b, false, nullptr, true)) continue;
+        for (int lg = LongCfg<T>::LGB; lg >= 0; --lg) {
+            const int B = 1 << lg;
+            if (!fits(a, B, NT, VMAX) || !fits(b, B, NT, VMAX) ||
+                (size_t)B * std::max(L1, L2) * sizeof(Cx<T>) > LONG_LDS)
+                continue;
+            const long d = std::labs((long)L1 - (long)L2);
+            if (lg > bestlg || (lg == bestlg && d < bestd)) {
+                best1 = L1;
+                bestlg = lg;
+                bestd = d;
+            }
+            break;
+        }
+    }
+    if (bestlg < 0) return "FFT length " + std::to_string(L) + " has no split for the four-step pass";
+    lp.L = L;
+    lp.L1 = best1;
+    lp.L2 = L / best1;
+    lp.lgb = bestlg;
+    factor_radices(lp.L1, lp.p1, false, nullptr, true);
+    factor_radices(lp.L2, lp.p2, false, nullptr, true);
+    std::string err;
+    lp.tw1 = upload_stage_twiddles<T>(lp.p1, err);
+    if (!lp.tw1) return err;
+    lp.tw2 = upload_stage_twiddles<T>(lp.p2, err);
+    if (!lp.tw2) return err;
+    lp.roots = upload_roots<T>(L, L, -1, err);
+    if (!lp.roots) return err;
+    lp.p1.tw = lp.tw1;
+    lp.p2.tw = lp.tw2;
+    return "";
+}
+template <class T> static void long_free(LongPlan<T> &lp) {
+    for (Cx<T> *p : {lp.tw1, lp.tw2, lp.roots})
+        if (p) (void)hipFree(p);
+    lp = LongPlan<T>();
+}
+
 template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (FW < 2 || FH < 1 || (FW & 1)) return "FFT width must be even and >= 2";
     s.FW = FW; s.FH = FH; s.M = FW / 2; s.NC = FW / 2 + 1;
-    if (!factor_radices(s.M, row, false, nullptr, row_odd_first)) return "FFT width/2 is not 7-smooth";
-    {
+    if (!seven_smooth_len(s.M)) return "FFT width/2 is not 7-smooth";
+    if (!seven_smooth_len(FH)) return "FFT height is not 7-smooth";
+    if ((size_t)s.NC * (size_t)FH > ((size_t)1 << 40)) return "FFT size too large";
+    // rows of RB lines in one workgroup's LDS and registers, else four-step rows
+    long_row = !factor_radices(s.M, row, false, nullptr, row_odd_first) || row_lds<T>(s.M) > LDS_BUDGET ||
+               !fits(row, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX);
+    if (!long_row) {
         // a radix-16 stage where it saves one and fits the row tiles' registers
         FftLen r16{};
         if (factor_radices(s.M, r16, false, nullptr, row_odd_first, true) && r16.nst < row.nst &&
             fits(r16, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX))
             row = r16;
     }
-    if (!factor_radices(FH, col, allow_edge, &col_re)) return "FFT height is not 7-smooth";
-    // the edge kernel holds EV / RE butterflies of each thread in registers
-    if (col_re && FH / col_re > (ColCfg<T>::EV / col_re) * ColCfg<T>::NT) col_re = 0;
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
-    if (row_lds<T>(s.M) > LDS_BUDGET || !fits(row, RowCfg<T>::RB, RowCfg<T>::NT, RowCfg<T>::VMAX))
-        return "FFT width too large for the LDS-resident row pass";
-    // the LDS-only column kernel keeps XMAX values per thread in registers; the
-    // edge-stage kernel covers the column with its edge registers instead
-    if ((size_t)FH * sizeof(Cx<T>) > LDS_BUDGET || (col_re == 0 && FH > CNT * CXMAX) || !fits(col, 1, CNT, CVMAX))
-        return "FFT height too large for the LDS-resident column pass";
+    // one column per workgroup, else four-step columns.  The edge kernel holds
+    // EV / RE butterflies of each thread in registers; the LDS-only column
+    // kernel keeps XMAX values per thread in registers
+    long_col = !factor_radices(FH, col, allow_edge, &col_re);
+    if (!long_col && col_re && FH / col_re > (ColCfg<T>::EV / col_re) * ColCfg<T>::NT) col_re = 0;
+    long_col = long_col || (size_t)FH * sizeof(Cx<T>) > LDS_BUDGET || (col_re == 0 && FH > CNT * CXMAX) ||
+               !fits(col, 1, CNT, CVMAX);
+    if (long_col) col_re = 0;
     std::string err;
-    tw_row = upload_stage_twiddles<T>(row, err);
-    if (!tw_row) return err;
-    tw_col = upload_stage_twiddles<T>(col, err);
-    if (!tw_col) return err;
+    if (long_row) {
+        if (!(err = long_plan<T>(s.M, lrow)).empty()) return err;
+    } else {
+        tw_row = upload_stage_twiddles<T>(row, err);
+        if (!tw_row) return err;
+        row.tw = tw_row;
+    }
+    if (long_col) {
+        if (!(err = long_plan<T>(FH, lcol)).empty()) return err;
+    } else {
+        tw_col = upload_stage_twiddles<T>(col, err);
+        if (!tw_col) return err;
+        col.tw = tw_col;
+    }
     post = upload_roots<T>(FW, s.M + 1, -1, err);
     if (!post) return err;
-    row.tw = tw_row;
-    col.tw = tw_col;
     const int rlds = (int)row_lds<T>(s.M);
     const int col_lds = (int)((size_t)FH * sizeof(Cx<T>));
     void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_c2r_p_fn<T>(),
@@ -990,11 +1245,14 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
             return "cannot query the device";
         row_grid = std::max(8, prop.multiProcessorCount / 8 * 8);
         constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT;
-        row_persistent = row_persistent && (s.M + 1) * RB <= RowCfg<T>::PFM * NT && s.M <= RowCfg<T>::PFR * NT;
+        row_persistent = row_persistent && !long_row && (s.M + 1) * RB <= RowCfg<T>::PFM * NT &&
+                         s.M <= RowCfg<T>::PFR * NT;
     }
-    for (void *fn : fns_row)
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, rlds) != hipSuccess)
-            return "cannot raise LDS limit for the row pass";
+    if (!long_row)
+        for (void *fn : fns_row)
+            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, rlds) != hipSuccess)
+                return "cannot raise LDS limit for the row pass";
+    if (long_col) return "";
     void *fns_col[] = {col_fn<T, T, COL_FWD>(), col_fn<T, T, COL_INV>(), col_fn<T, T, COL_BCAST>(),
                        col_fn<T, float, COL_ADJ>(), col_fn<T, double, COL_ADJ>()};
     for (void *fn : fns_col)
@@ -1024,6 +1282,15 @@ template <class T> void Fft2<T>::release() {
     if (tw_col) (void)hipFree(tw_col);
     if (post) (void)hipFree(post);
     tw_row = tw_col = post = nullptr;
+    long_free(lrow);
+    long_free(lcol);
+    long_row = long_col = false;
+    if (lt_a) (void)hipFree(lt_a);
+    if (lt_b) (void)hipFree(lt_b);
+    if (lt_c) (void)hipFree(lt_c);
+    lt_a = lt_b = nullptr;
+    lt_c = nullptr;
+    lt_a_n = lt_b_n = lt_c_bytes = 0;
 }
 
 template <class T>
@@ -1031,6 +1298,9 @@ hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pit
                             const T *wr, int W, Cx<T> *out, size_t out_bstride, int rp_rows, int rows, int batch,
                             hipStream_t st) {
     if (rows <= 0 || batch <= 0) return hipSuccess;
+    if (long_row)
+        return row_r2c_long(mode, in, in_bstride, in_pitch, win, ex, wr, W, out, out_bstride, rp_rows, rows, batch,
+                            st);
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     const size_t lds = row_lds<T>(s.M);
     dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
@@ -1053,6 +1323,7 @@ template <class T>
 hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *out, size_t out_bstride,
                             int out_pitch, int wout, T scale, int rows, int batch, hipStream_t st) {
     if (rows <= 0 || batch <= 0) return hipSuccess;
+    if (long_row) return row_c2r_long(in, in_bstride, rp_rows, out, out_bstride, out_pitch, wout, scale, rows, batch, st);
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     const size_t lds = row_lds<T>(s.M);
     const int vec = pairs_aligned<T>(out, out_pitch, out_bstride);
@@ -1096,6 +1367,11 @@ template <class TA>
 hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
                                  size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
                                  int hout, int batch, FftLen fa, size_t lds, hipStream_t st) {
+    if (long_col) {
+        if (fa.L > 0) return hipErrorInvalidValue;  // no fused TA inverse on four-step columns
+        return colpass_long<TA>(mode, in, in_bstride, in_rows, out, out_bstride, out_rows, psf, psf_bstride, hin,
+                                hout, batch, st);
+    }
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX, CXMAX = ColCfg<T>::XMAX;
     const int gx = (s.NC + 7) / 8 * 8;
     hin = std::min(hin, s.FH);
@@ -1153,7 +1429,7 @@ hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, i
 
 template <class T> bool Fft2<T>::acc_inverse_ok(const FftLen &fa) const {
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX;
-    if (fa.L != s.FH || (size_t)s.FH * sizeof(Cx<double>) > LDS_BUDGET || !fits(fa, 1, CNT, CVMAX)) return false;
+    if (long_col || !fa.tw || fa.L != s.FH || (size_t)s.FH * sizeof(Cx<double>) > LDS_BUDGET || !fits(fa, 1, CNT, CVMAX)) return false;
     if (!col_re) return true;  // the LDS-only kernel runs plan fa as it is
     // the edge kernel keeps its own edge radix: plan fa must have the same radices
     if (fa.nst != col.nst) return false;
@@ -1197,6 +1473,261 @@ hipError_t Fft2<T>::spec_cast(const Cx<TA> *acc, Cx<T> *out, hipStream_t st) {
     hipLaunchKernelGGL((k_spec<T, TA, 2>), dim3(g), dim3(256), 0, st, (const Cx<T> *)nullptr, (const Cx<T> *)nullptr,
                        out, (Cx<TA> *)acc, rp(s.FH), s.NC, s.FH);
     return hipGetLastError();
+}
+
+template <class T> hipError_t Fft2<T>::long_space(size_t a, size_t b, size_t cbytes, hipStream_t st) {
+    if (a <= lt_a_n && b <= lt_b_n && cbytes <= lt_c_bytes) return hipSuccess;
+    // a captured launch sequence cannot allocate: long_reserve sizes the
+    // scratch for the pipeline's calls beforehand
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return hipErrorOutOfMemory;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    auto grow = [](void **p, size_t &have, size_t need, size_t unit) -> hipError_t {
+        if (need <= have) return hipSuccess;
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+        have = 0;
+        const hipError_t r = hipMalloc(p, need * unit);
+        if (r != hipSuccess) {
+            *p = nullptr;
+            return r;
+        }
+        have = need;
+        return hipSuccess;
+    };
+    if ((e = grow((void **)&lt_a, lt_a_n, a, sizeof(Cx<T>))) != hipSuccess) return e;
+    if ((e = grow((void **)&lt_b, lt_b_n, b, sizeof(Cx<T>))) != hipSuccess) return e;
+    return grow(&lt_c, lt_c_bytes, cbytes, 1);
+}
+
+template <class T> hipError_t Fft2<T>::long_reserve(int nexp) {
+    const size_t n = (size_t)std::max(nexp, 1);
+    size_t a = 0, b = 0, cb = 0;
+    if (long_row) a = b = n * (size_t)s.FH * (size_t)s.M;  // up to n images of FH rows
+    if (long_col) {
+        const size_t sp = n * (size_t)s.NC * (size_t)s.FH;  // n full spectra
+        a = std::max(a, sp);
+        b = std::max(b, sp);
+        cb = (size_t)s.NC * s.FH * sizeof(Cx<double>);
+    }
+    return long_space(a, b, cb, nullptr);
+}
+
+template <class T, class TS> static void *long_a_fn(bool inv, int ld) {
+    constexpr int NT = LongCfg<T>::NT, V = LongCfg<T>::VMAX;
+    if constexpr (!std::is_same<T, TS>::value) {
+        // a wider source: only the summed spectrum, read as T
+        if (ld != LL_CX) return nullptr;
+        return inv ? (void *)k_long_a<T, TS, true, LL_CX, NT, V> : (void *)k_long_a<T, TS, false, LL_CX, NT, V>;
+    } else {
+        switch (ld) {
+            case LL_CX:
+                return inv ? (void *)k_long_a<T, T, true, LL_CX, NT, V> : (void *)k_long_a<T, T, false, LL_CX, NT, V>;
+            case LL_CX_MUL:
+                return inv ? (void *)k_long_a<T, T, true, LL_CX_MUL, NT, V>
+                           : (void *)k_long_a<T, T, false, LL_CX_MUL, NT, V>;
+            case LL_REAL:
+                return inv ? (void *)k_long_a<T, T, true, LL_REAL, NT, V>
+                           : (void *)k_long_a<T, T, false, LL_REAL, NT, V>;
+            case LL_GATHER:
+                return inv ? (void *)k_long_a<T, T, true, LL_GATHER, NT, V>
+                           : (void *)k_long_a<T, T, false, LL_GATHER, NT, V>;
+            default: return nullptr;
+        }
+    }
+}
+template <class T> static void *long_b_fn(bool inv, int store) {
+    constexpr int NT = LongCfg<T>::NT, V = LongCfg<T>::VMAX;
+    if (store == LS_CX) return inv ? (void *)k_long_b<T, true, LS_CX, NT, V> : (void *)k_long_b<T, false, LS_CX, NT, V>;
+    return inv ? (void *)k_long_b<T, true, LS_REAL, NT, V> : (void *)k_long_b<T, false, LS_REAL, NT, V>;
+}
+
+// Workgroups per launch stay below 2^22; longer line sets take several launches.
+static constexpr size_t LONG_MAX_WG = (size_t)1 << 22;
+
+template <class T>
+template <class TS>
+hipError_t Fft2<T>::long_a(const LongPlan<T> &lp, bool inv, int ld, const LineIO &src, size_t lines, Cx<T> *tmp,
+                           hipStream_t st) {
+    void *fn = long_a_fn<T, TS>(inv, ld);
+    if (!fn || lp.L == 0) return hipErrorInvalidValue;
+    FftLen p2 = lp.p2;
+    const Cx<T> *roots = lp.roots;
+    int L1 = lp.L1, lgb = lp.lgb, nblk = (lp.L1 + (1 << lp.lgb) - 1) >> lp.lgb;
+    LineIO io = src;
+    Cx<T> *t = tmp;
+    const size_t lds = ((size_t)lp.L2 << lp.lgb) * sizeof(Cx<T>);
+    const size_t per = std::max<size_t>(1, LONG_MAX_WG / (size_t)nblk);
+    for (size_t l0 = 0; l0 < lines; l0 += per) {
+        size_t line0 = l0;
+        const size_t nl = std::min(per, lines - l0);
+        void *args[] = {(void *)&p2, (void *)&roots, (void *)&L1, (void *)&lgb, (void *)&nblk, (void *)&line0,
+                        (void *)&io, (void *)&t};
+        const hipError_t e = hipLaunchKernel(fn, dim3((unsigned)(nl * nblk)), dim3(LongCfg<T>::NT), args, lds, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <class T>
+hipError_t Fft2<T>::long_b(const LongPlan<T> &lp, bool inv, int store, const Cx<T> *tmp, const LineIO &dst,
+                           size_t lines, hipStream_t st) {
+    void *fn = long_b_fn<T>(inv, store);
+    if (lp.L == 0) return hipErrorInvalidValue;
+    FftLen p1 = lp.p1;
+    int L2 = lp.L2, lgb = lp.lgb, nblk = (lp.L2 + (1 << lp.lgb) - 1) >> lp.lgb;
+    const Cx<T> *t = tmp;
+    LineIO io = dst;
+    const size_t lds = ((size_t)lp.L1 << lp.lgb) * sizeof(Cx<T>);
+    const size_t per = std::max<size_t>(1, LONG_MAX_WG / (size_t)nblk);
+    for (size_t l0 = 0; l0 < lines; l0 += per) {
+        size_t line0 = l0;
+        const size_t nl = std::min(per, lines - l0);
+        void *args[] = {(void *)&p1, (void *)&L2, (void *)&lgb, (void *)&nblk, (void *)&line0, (void *)&t,
+                        (void *)&io};
+        const hipError_t e = hipLaunchKernel(fn, dim3((unsigned)(nl * nblk)), dim3(LongCfg<T>::NT), args, lds, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Row R2C of rows longer than one workgroup: pass A reads the real rows as
+// pairs (or gathers them), pass B leaves natural-order half-length
+// transforms, the split kernel writes the column-major half spectra.
+template <class T>
+hipError_t Fft2<T>::row_r2c_long(int mode, const T *in, size_t in_bstride, int in_pitch, int win, const ExpDev *ex,
+                                 const T *wr, int W, Cx<T> *out, size_t out_bstride, int rp_rows, int rows, int batch,
+                                 hipStream_t st) {
+    const size_t lines = (size_t)rows * batch, M = (size_t)s.M;
+    if (lines > (size_t)INT32_MAX) return hipErrorInvalidValue;
+    hipError_t e = long_space(lines * M, lines * M, 0, st);
+    if (e != hipSuccess) return e;
+    LineIO src;
+    src.p = in;
+    src.bstride = in_bstride;
+    src.pitch = (size_t)in_pitch;
+    src.lpb = rows;
+    src.nvalid = mode == ROW_PLAIN ? win : W;
+    src.ex = ex;
+    src.wr = wr;
+    if ((e = long_a<T>(lrow, false, mode == ROW_PLAIN ? LL_REAL : LL_GATHER, src, lines, lt_a, st)) != hipSuccess)
+        return e;
+    LineIO mid;
+    mid.p = lt_b;
+    mid.pitch = M;
+    mid.lpb = (int)lines;
+    mid.nvalid = s.M;
+    if ((e = long_b(lrow, false, LS_CX, lt_a, mid, lines, st)) != hipSuccess) return e;
+    const dim3 g((s.M + 1 + 63) / 64, (rows + 15) / 16, batch);
+    hipLaunchKernelGGL(k_long_split<T>, g, dim3(256), 0, st, lt_b, s.M, rows, post, out, out_bstride, rp(rp_rows));
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t Fft2<T>::row_c2r_long(const Cx<T> *in, size_t in_bstride, int rp_rows, T *out, size_t out_bstride,
+                                 int out_pitch, int wout, T scale, int rows, int batch, hipStream_t st) {
+    const size_t lines = (size_t)rows * batch, M = (size_t)s.M;
+    if (lines > (size_t)INT32_MAX) return hipErrorInvalidValue;
+    hipError_t e = long_space(lines * M, lines * M, 0, st);
+    if (e != hipSuccess) return e;
+    const dim3 g((s.M + 63) / 64, (rows + 15) / 16, batch);
+    hipLaunchKernelGGL(k_long_merge<T>, g, dim3(256), 0, st, in, in_bstride, rp(rp_rows), s.M, rows, post, lt_b);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    LineIO z;
+    z.p = lt_b;
+    z.pitch = M;
+    z.lpb = (int)lines;
+    z.nvalid = s.M;
+    if ((e = long_a<T>(lrow, true, LL_CX, z, lines, lt_a, st)) != hipSuccess) return e;
+    LineIO dst;
+    dst.p = out;
+    dst.bstride = out_bstride;
+    dst.pitch = (size_t)out_pitch;
+    dst.lpb = rows;
+    dst.nvalid = wout;
+    dst.scale = (double)scale;
+    return long_b(lrow, true, LS_REAL, lt_a, dst, lines, st);
+}
+
+// Column passes over columns longer than one workgroup, composed from the
+// two passes: BCAST as X^ = FFT(in) into scratch, then inverse passes whose
+// loads multiply by each PSF spectrum; ADJ / ACC as forward passes of every
+// exposure into scratch, the summed product with conj(P^_k), and (ADJ) its
+// inverse.
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::colpass_long(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
+                                 size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
+                                 int hout, int batch, hipStream_t st) {
+    if (batch < 0) return hipErrorInvalidValue;
+    const int NC = s.NC, FH = s.FH;
+    const size_t L = (size_t)FH, nl = (size_t)NC, nb = (size_t)batch;
+    if (nb * nl > (size_t)INT32_MAX) return hipErrorInvalidValue;
+    LineIO src;
+    src.p = in;
+    src.bstride = in_bstride;
+    src.pitch = (size_t)rp(in_rows);
+    src.lpb = NC;
+    src.nvalid = std::min(hin, FH);
+    LineIO dst;
+    dst.p = out;
+    dst.bstride = out_bstride;
+    dst.pitch = (size_t)rp(out_rows);
+    dst.lpb = NC;
+    dst.nvalid = std::min(hout, FH);
+    LineIO full;  // line-major scratch, FH elements per line
+    full.p = lt_b;
+    full.pitch = L;
+    full.nvalid = FH;
+    hipError_t e;
+    switch (mode) {
+        case COL_FWD:
+        case COL_INV: {
+            if ((e = long_space(nb * nl * L, 0, 0, st)) != hipSuccess) return e;
+            const bool inv = mode == COL_INV;
+            if ((e = long_a<T>(lcol, inv, LL_CX, src, nb * nl, lt_a, st)) != hipSuccess) return e;
+            return long_b(lcol, inv, LS_CX, lt_a, dst, nb * nl, st);
+        }
+        case COL_BCAST: {
+            if (nb == 0) return hipSuccess;
+            if ((e = long_space(nb * nl * L, nl * L, 0, st)) != hipSuccess) return e;
+            src.bstride = 0;
+            if ((e = long_a<T>(lcol, false, LL_CX, src, nl, lt_a, st)) != hipSuccess) return e;
+            full.p = lt_b;
+            full.lpb = NC;
+            if ((e = long_b(lcol, false, LS_CX, lt_a, full, nl, st)) != hipSuccess) return e;
+            LineIO mul = full;
+            mul.psf = psf;
+            mul.pbstride = psf_bstride;
+            mul.ppitch = (size_t)rp(FH);
+            if ((e = long_a<T>(lcol, true, LL_CX_MUL, mul, nb * nl, lt_a, st)) != hipSuccess) return e;
+            return long_b(lcol, true, LS_CX, lt_a, dst, nb * nl, st);
+        }
+        case COL_ADJ:
+        case COL_ACC: {
+            if ((e = long_space(nb * nl * L, nb * nl * L, nl * L * sizeof(Cx<TA>), st)) != hipSuccess) return e;
+            if ((e = long_a<T>(lcol, false, LL_CX, src, nb * nl, lt_a, st)) != hipSuccess) return e;
+            full.p = lt_b;
+            full.lpb = (int)std::max<size_t>(nb * nl, 1);
+            if ((e = long_b(lcol, false, LS_CX, lt_a, full, nb * nl, st)) != hipSuccess) return e;
+            Cx<TA> *acc = mode == COL_ACC ? reinterpret_cast<Cx<TA> *>(out) : reinterpret_cast<Cx<TA> *>(lt_c);
+            const int opitch = mode == COL_ACC ? rp(out_rows) : FH;
+            const unsigned g = (unsigned)std::min<size_t>((nl * L + 255) / 256, 16384);
+            hipLaunchKernelGGL((k_long_adjsum<T, TA>), dim3(g), dim3(256), 0, st, lt_b, NC, FH, batch, psf,
+                               psf_bstride, rp(FH), acc, opitch);
+            if ((e = hipGetLastError()) != hipSuccess || mode == COL_ACC) return e;
+            LineIO sum;
+            sum.p = lt_c;
+            sum.pitch = L;
+            sum.lpb = NC;
+            sum.nvalid = FH;
+            if ((e = long_a<TA>(lcol, true, LL_CX, sum, nl, lt_a, st)) != hipSuccess) return e;
+            dst.bstride = 0;
+            return long_b(lcol, true, LS_CX, lt_a, dst, nl, st);
+        }
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template struct Fft2<float>;

@@ -26,6 +26,35 @@ struct FftShape {
 // Column pitch (elements) of a spectrum holding `rows` rows.
 inline int rp(int rows) { return (rows + 15) / 16 * 16; }
 
+// Lines too long for one workgroup's LDS (rows of more than M = 4096 points,
+// columns of more than 8192) run as a four-step transform: L = L1 x L2, index
+// n = n1 + L1 n2 in, k = k2 + L2 k1 out.  Pass A: for each n1, the L2-point
+// transform over n2 of the strided inputs, times exp(-+2 pi i n1 k2 / L),
+// stored to a line-major scratch at k2 L1 + n1.  Pass B: for each k2, the
+// L1-point transform over n1, stored at k2 + L2 k1.  Both passes transform
+// B adjacent sub-lines per workgroup (interleaved in LDS), so every global
+// access moves B consecutive elements.
+enum LongLoad { LL_CX = 0, LL_CX_MUL = 1, LL_REAL = 2, LL_GATHER = 3 };
+enum LongStore { LS_CX = 0, LS_REAL = 1 };
+struct LineIO {
+    const void *p = nullptr;  // source or destination base
+    size_t bstride = 0;       // elements between batches (lpb lines each)
+    size_t pitch = 0;         // elements between the lines of one batch
+    int lpb = 1;              // lines per batch
+    int nvalid = 0;           // loads: elements read (LL_REAL: real columns, LL_GATHER: W), zero beyond;
+                              // stores: elements kept (LS_REAL: real columns)
+    const void *psf = nullptr;  // LL_CX_MUL: line l of batch b times psf[b pbstride + l ppitch + n]
+    size_t pbstride = 0, ppitch = 0;
+    const ExpDev *ex = nullptr;  // LL_GATHER: exposure b; wr per source pixel
+    const void *wr = nullptr;
+    double scale = 1.0;  // LS_REAL
+};
+template <class T> struct LongPlan {
+    int L = 0, L1 = 0, L2 = 0, lgb = 0;  // B = 1 << lgb sub-lines per workgroup
+    FftLen p1{}, p2{};                    // L1- and L2-point plans
+    Cx<T> *tw1 = nullptr, *tw2 = nullptr, *roots = nullptr;  // roots[m] = exp(-2 pi i m / L)
+};
+
 template <class T> struct Fft2 {
     FftShape s;
     FftLen row{}, col{};
@@ -35,10 +64,22 @@ template <class T> struct Fft2 {
     bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit
     bool row_odd_first = true;   // set before init(): an odd radix first in the row plan (LDS banks)
     int row_grid = 0;            // workgroups of the persistent row passes (one per CU)
+    // Four-step rows / columns where one workgroup cannot hold a line, with
+    // their device scratch (a, b: complex T line sets; c: one summed spectrum
+    // in the accumulation type).
+    bool long_row = false, long_col = false;
+    LongPlan<T> lrow, lcol;
+    Cx<T> *lt_a = nullptr, *lt_b = nullptr;
+    void *lt_c = nullptr;
+    size_t lt_a_n = 0, lt_b_n = 0, lt_c_bytes = 0;
 
     // Returns "" on success or an error message.
     std::string init(int FW, int FH);
     void release();
+    // Four-step scratch for transforms of up to `nexp` images (or spectra)
+    // at the full FFT size, allocated now rather than at the first call (a
+    // call under graph capture cannot allocate).  No-op when every line fits.
+    hipError_t long_reserve(int nexp);
     size_t spec_elems(int rows) const { return (size_t)s.NC * rp(rows); }
 
     // Row R2C of rows [0, rows) of `batch` images -> column-major spectra
@@ -91,6 +132,23 @@ template <class T> struct Fft2 {
     hipError_t spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st);
     template <class TA> hipError_t spec_acc(const Cx<T> *a, const Cx<T> *psf_k, Cx<TA> *acc, hipStream_t st);
     template <class TA> hipError_t spec_cast(const Cx<TA> *acc, Cx<T> *out, hipStream_t st);
+
+    // four-step pieces (fft.hip)
+    hipError_t long_space(size_t a, size_t b, size_t cbytes, hipStream_t st);
+    template <class TS>
+    hipError_t long_a(const LongPlan<T> &lp, bool inv, int ld, const LineIO &src, size_t lines, Cx<T> *tmp,
+                      hipStream_t st);
+    hipError_t long_b(const LongPlan<T> &lp, bool inv, int store, const Cx<T> *tmp, const LineIO &dst, size_t lines,
+                      hipStream_t st);
+    hipError_t row_r2c_long(int mode, const T *in, size_t in_bstride, int in_pitch, int win, const ExpDev *ex,
+                            const T *wr, int W, Cx<T> *out, size_t out_bstride, int rp_rows, int rows, int batch,
+                            hipStream_t st);
+    hipError_t row_c2r_long(const Cx<T> *in, size_t in_bstride, int rp_rows, T *out, size_t out_bstride,
+                            int out_pitch, int wout, T scale, int rows, int batch, hipStream_t st);
+    template <class TA>
+    hipError_t colpass_long(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
+                            size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
+                            int hout, int batch, hipStream_t st);
 };
 
 }  // namespace updc

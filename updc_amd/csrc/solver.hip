@@ -542,7 +542,9 @@ template <class TF, class TA> struct Pipe : PipeBase {
             fa.release();
             e = fa.init(c.FW, c.FH);
             if (!e.empty()) throw Error(UPDC_ERR_ARG, e);
+            HIPCHK(fa.long_reserve(n));
         }
+        HIPCHK(fft.long_reserve(n));
         c.setup.fft_plan = lap();
         const FftShape &s = fft.s;
         max_nin = 0;
