@@ -304,12 +304,14 @@ __device__ __forceinline__ void edge_last(Cx<T> (&rb)[NBE][RE], const Cx<T> *buf
 
 // Full transform over the LDS buffer (natural order in, natural order out):
 // all `1 << lg` interleaved lines together (cfix < 0) or line cfix only.
+// fft_lds_t reads the stage twiddles from twb (the tables of p copied
+// elsewhere, e.g. into LDS) instead of p.tw.
 template <bool INV, class T, int NT, int VMAX>
-__device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg, int cfix = -1) {
+__device__ __forceinline__ void fft_lds_t(Cx<T> *buf, const FftLen &p, const Cx<T> *twb, int lg, int cfix = -1) {
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.rad[s], Ns = p.ns[s];
         const unsigned int mg = p.mg[s];
-        const Cx<T> *tw = (const Cx<T> *)p.tw + p.to[s];
+        const Cx<T> *tw = twb + p.to[s];
         switch (R) {
             case 2: fft_stage<2, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
             case 3: fft_stage<3, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
@@ -322,6 +324,10 @@ __device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg, int
             default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
         }
     }
+}
+template <bool INV, class T, int NT, int VMAX>
+__device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg, int cfix = -1) {
+    fft_lds_t<INV, T, NT, VMAX>(buf, p, (const Cx<T> *)p.tw, lg, cfix);
 }
 
 // ---------------------------------------------------------------- XCD-aware tiles

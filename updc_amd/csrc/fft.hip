@@ -225,11 +225,37 @@ __device__ __forceinline__ void row_tiles(int ntiles, int &t, int &t_end, int &s
     t_end = min(ntiles, (xcd + 1) * tper);
 }
 
+// The persistent C2R workgroups copy the stage twiddles, and the split roots
+// where PL, into LDS behind the tile, once.  Read from global memory, each
+// table load would make its wave wait for the tile prefetch in flight as well
+// (vector memory loads retire in order), and the prefetch would overlap
+// little (0.71 -> 0.64 ms for 8 exposures of 4096 rows on a 5600-point grid).
+// The persistent R2C keeps its tables in global memory: there the extra LDS
+// reads cost more than the overlap gains (0.46 -> 0.64 ms measured).  FP64
+// rows leave the roots in global memory (no LDS room).
+template <class T, int NT, bool PL>
+__device__ __forceinline__ void row_tables_lds(const FftLen &f, const Cx<T> *__restrict__ post, int ntw,
+                                               Cx<T> *lds, const Cx<T> *&twl, const Cx<T> *&pl) {
+    const Cx<T> *tw = (const Cx<T> *)f.tw;
+    for (int i = threadIdx.x; i < ntw; i += NT) lds[i] = tw[i];
+    twl = lds;
+    if constexpr (PL) {
+        Cx<T> *pp = lds + ntw;
+        for (int i = threadIdx.x; i <= f.L; i += NT) pp[i] = post[i];
+        pl = pp;
+    } else {
+        pl = post;
+    }
+}
+template <class T> struct RowPL;
+template <> struct RowPL<float> { static constexpr bool value = true; };
+template <> struct RowPL<double> { static constexpr bool value = false; };
+
 // C2R tiles of a persistent workgroup (as k_row_c2r).  PFM complex values per
 // thread cover (M + 1) * RB slots.  emit(b, r0, z) consumes each transformed
 // tile from LDS (z[(n << LG) + j] = columns 2n, 2n+1 of row r0 + j, unscaled).
 template <class T, int NT, int VMAX, int RB, int PFM, class Emit>
-__device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *__restrict__ post,
+__device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *twl, const Cx<T> *__restrict__ post,
                                               const Cx<T> *__restrict__ in, size_t in_bstride, int RP, int rows,
                                               int batch, Cx<T> *z, Emit &emit) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
@@ -276,19 +302,21 @@ __device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *__re
             z[(v << LG) + j] = cv;
         }
         __syncthreads();
-        fft_lds<true, T, NT, VMAX>(z, f, LG);
+        fft_lds_t<true, T, NT, VMAX>(z, f, twl, LG);
         emit(b, r0, z);
     }
 }
 
 template <class T, int NT, int VMAX, int RB, int PFM>
 __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
-    FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in, size_t in_bstride, int RP,
+    FftLen f, const Cx<T> *__restrict__ post, int ntw, const Cx<T> *__restrict__ in, size_t in_bstride, int RP,
     T *__restrict__ out, size_t out_bstride, int out_pitch, int wout, T scale, int rows, int batch, int vec) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = f.L;
+    const Cx<T> *twl, *pl;
+    row_tables_lds<T, NT, RowPL<T>::value>(f, post, ntw, z + ((M + 1) << LG), twl, pl);
     auto emit = [&](int b, int r0, const Cx<T> *zz) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
@@ -301,7 +329,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
             }
         }
     };
-    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, post, in, in_bstride, RP, rows, batch, z, emit);
+    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, twl, pl, in, in_bstride, RP, rows, batch, z, emit);
 }
 
 // Persistent C2R of the summed adjoint spectrum (batch 1) with the update
@@ -313,13 +341,15 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
 // adds the workgroup partials in workgroup order.
 template <class T, class TF, int NT, int VMAX, int RB, int PFM>
 __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
-    FftLen f, const Cx<T> *__restrict__ post, const Cx<T> *__restrict__ in, int RP, TF *__restrict__ x,
+    FftLen f, const Cx<T> *__restrict__ post, int ntw, const Cx<T> *__restrict__ in, int RP, TF *__restrict__ x,
     const T *__restrict__ D, int W, T scale, T max_sens, int rows, IterDiag *idiag, double *partials,
     unsigned long long *negs, unsigned int *ticket) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = f.L;
+    const Cx<T> *twl, *pl;
+    row_tables_lds<T, NT, RowPL<T>::value>(f, post, ntw, z + ((M + 1) << LG), twl, pl);
     const T thr = sizeof(T) == 4 ? (T)1e-8f : (T)1e-8;
     const T neg = sizeof(T) == 4 ? (T)(-1.0e-6f) : (T)(-1.0e-6);
     double sum = 0.0;
@@ -368,7 +398,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
             }
         }
     };
-    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, post, in, 0, RP, rows, 1, z, emit);
+    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, twl, pl, in, 0, RP, rows, 1, z, emit);
 
     if (bad) {
         atomicOr(&idiag->err, bad);
@@ -1101,6 +1131,17 @@ template <class T> static void *row_c2r_fn() {
 // LDS of a row workgroup: RB interleaved rows of M points, plus the Nyquist
 // column the C2R merge stages.
 template <class T> static size_t row_lds(int M) { return (size_t)RowCfg<T>::RB * (M + 1) * sizeof(Cx<T>); }
+// ... and of a persistent C2R workgroup: the tile, the stage twiddles and
+// (FP32) the split roots
+template <class T> static size_t row_lds_p(int M, int ntw) {
+    return row_lds<T>(M) + ((size_t)ntw + (RowPL<T>::value ? (size_t)M + 1 : 0)) * sizeof(Cx<T>);
+}
+// entries of a plan's stage twiddle tables (as upload_stage_twiddles lays them out)
+static int stage_twiddle_count(const FftLen &f) {
+    long n = 0;
+    for (int s = 0; s < f.nst; ++s) n += (long)f.ns[s] * (f.rad[s] - 1);
+    return (int)std::max<long>(n, 1);
+}
 template <class T, class TA, int MODE> static void *col_fn() {
     return (void *)k_col<T, TA, ColCfg<T>::NT, ColCfg<T>::VMAX, ColCfg<T>::XMAX, MODE>;
 }
@@ -1223,6 +1264,7 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
         tw_row = upload_stage_twiddles<T>(row, err);
         if (!tw_row) return err;
         row.tw = tw_row;
+        row_ntw = stage_twiddle_count(row);
     }
     if (long_col) {
         if (!(err = long_plan<T>(FH, lcol)).empty()) return err;
@@ -1233,10 +1275,10 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     }
     post = upload_roots<T>(FW, s.M + 1, -1, err);
     if (!post) return err;
-    const int rlds = (int)row_lds<T>(s.M);
+    const int rlds = (int)row_lds<T>(s.M), rlds_p = (int)row_lds_p<T>(s.M, row_ntw);
     const int col_lds = (int)((size_t)FH * sizeof(Cx<T>));
-    void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_c2r_p_fn<T>(),
-                       row_r2c_p_fn<T>(), row_c2r_upd_fn<T, float>(), row_c2r_upd_fn<T, T>()};
+    void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_r2c_p_fn<T>()};
+    void *fns_row_p[] = {row_c2r_p_fn<T>(), row_c2r_upd_fn<T, float>(), row_c2r_upd_fn<T, T>()};
     // persistent row passes: one workgroup per CU, prefetch registers must cover a tile
     {
         int dev = 0;
@@ -1246,11 +1288,15 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
         row_grid = std::max(8, prop.multiProcessorCount / 8 * 8);
         constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT;
         row_persistent = row_persistent && !long_row && (s.M + 1) * RB <= RowCfg<T>::PFM * NT &&
-                         s.M <= RowCfg<T>::PFR * NT;
+                         s.M <= RowCfg<T>::PFR * NT && (size_t)rlds_p <= LDS_BUDGET;
     }
     if (!long_row)
         for (void *fn : fns_row)
             if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, rlds) != hipSuccess)
+                return "cannot raise LDS limit for the row pass";
+    if (row_persistent)
+        for (void *fn : fns_row_p)
+            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, rlds_p) != hipSuccess)
                 return "cannot raise LDS limit for the row pass";
     if (long_col) return "";
     void *fns_col[] = {col_fn<T, T, COL_FWD>(), col_fn<T, T, COL_INV>(), col_fn<T, T, COL_BCAST>(),
@@ -1328,9 +1374,9 @@ hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *
     const size_t lds = row_lds<T>(s.M);
     const int vec = pairs_aligned<T>(out, out_pitch, out_bstride);
     if (row_persistent) {
-        hipLaunchKernelGGL((k_row_c2r_p<T, NT, VMAX, RB, RowCfg<T>::PFM>), dim3(row_grid), dim3(NT), lds, st, row,
-                           post, in, in_bstride, rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows, batch,
-                           vec);
+        hipLaunchKernelGGL((k_row_c2r_p<T, NT, VMAX, RB, RowCfg<T>::PFM>), dim3(row_grid), dim3(NT),
+                           row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, in_bstride, rp(rp_rows), out,
+                           out_bstride, out_pitch, wout, scale, rows, batch, vec);
         return hipGetLastError();
     }
     const dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
@@ -1347,8 +1393,8 @@ hipError_t Fft2<T>::row_c2r_update(const Cx<T> *in, int rp_rows, TF *x, const T 
     if (!row_persistent) return hipErrorNotSupported;
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     hipLaunchKernelGGL((k_row_c2r_upd_p<T, TF, NT, VMAX, RB, RowCfg<T>::PFM>), dim3(row_grid), dim3(NT),
-                       row_lds<T>(s.M), st, row, post, in, rp(rp_rows), x, D, W, scale, max_sens, H, idiag, partials,
-                       negs, ticket);
+                       row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, rp(rp_rows), x, D, W, scale, max_sens,
+                       H, idiag, partials, negs, ticket);
     return hipGetLastError();
 }
 

@@ -64,6 +64,7 @@ template <class T> struct Fft2 {
     bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit
     bool row_odd_first = true;   // set before init(): an odd radix first in the row plan (LDS banks)
     int row_grid = 0;            // workgroups of the persistent row passes (one per CU)
+    int row_ntw = 0;             // entries of the row plan's stage twiddle tables
     // Four-step rows / columns where one workgroup cannot hold a line, with
     // their device scratch (a, b: complex T line sets; c: one summed spectrum
     // in the accumulation type).
