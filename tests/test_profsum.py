@@ -45,6 +45,13 @@ def test_stage_order_fused_and_split():
     assert got == FUSED_STAGES + SPLIT_STAGES
 
 
+def test_compact_adjoint_image_kernel_is_the_adjoint_stage():
+    # the compact-map gather (k_csc_image16) is the same stage as the 8-byte one
+    split16 = [n.replace("k_csc_image<", "k_csc_image16<") for n in SPLIT]
+    rows = [{"Dispatch_Id": str(i + 1), "Kernel_Name": n} for i, n in enumerate(SETUP + split16)]
+    assert [st for st, _ in P._iteration_stages(rows, "Kernel_Name")] == SPLIT_STAGES
+
+
 def test_setup_kernels_before_the_first_gather_are_skipped():
     # the set-up's own plain R2C and column FFT precede the window
     rows = [{"Dispatch_Id": str(i + 1), "Kernel_Name": n} for i, n in enumerate(SETUP + FUSED)]

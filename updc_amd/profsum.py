@@ -64,7 +64,7 @@ def classify(name: str, prev: str | None) -> str | None:
         return {"1": "col_inv_corr", "2": "col_fwd_bcast", "3": "col_adj_accum", "4": "col_adj_accum"}.get(mode)
     if "k_gather<" in name:
         return "gather_freeze"
-    if "k_csc_image<" in name:
+    if "k_csc_image<" in name or "k_csc_image16<" in name:
         return "adjoint_image"
     if "k_update<" in name:
         return "update"
