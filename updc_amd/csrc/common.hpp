@@ -304,10 +304,11 @@ __device__ __forceinline__ void edge_last(Cx<T> (&rb)[NBE][RE], const Cx<T> *buf
 
 // Full transform over the LDS buffer (natural order in, natural order out):
 // all `1 << lg` interleaved lines together (cfix < 0) or line cfix only.
-// fft_lds_t reads the stage twiddles from twb (the tables of p copied
-// elsewhere, e.g. into LDS) instead of p.tw.
-template <bool INV, class T, int NT, int VMAX>
-__device__ __forceinline__ void fft_lds_t(Cx<T> *buf, const FftLen &p, const Cx<T> *twb, int lg, int cfix = -1) {
+// fft_lds_hook reads the stage twiddles from twb (the tables of p copied
+// elsewhere, e.g. into LDS) instead of p.tw, and calls hook(s) after stage s.
+template <bool INV, class T, int NT, int VMAX, class Hook>
+__device__ __forceinline__ void fft_lds_hook(Cx<T> *buf, const FftLen &p, const Cx<T> *twb, int lg, int cfix,
+                                             Hook &hook) {
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.rad[s], Ns = p.ns[s];
         const unsigned int mg = p.mg[s];
@@ -323,7 +324,13 @@ __device__ __forceinline__ void fft_lds_t(Cx<T> *buf, const FftLen &p, const Cx<
                 break;
             default: fft_stage<8, INV, T, NT, VMAX>(buf, p.L, lg, cfix, Ns, mg, tw); break;
         }
+        hook(s);
     }
+}
+template <bool INV, class T, int NT, int VMAX>
+__device__ __forceinline__ void fft_lds_t(Cx<T> *buf, const FftLen &p, const Cx<T> *twb, int lg, int cfix = -1) {
+    auto none = [](int) {};
+    fft_lds_hook<INV, T, NT, VMAX>(buf, p, twb, lg, cfix, none);
 }
 template <bool INV, class T, int NT, int VMAX>
 __device__ __forceinline__ void fft_lds(Cx<T> *buf, const FftLen &p, int lg, int cfix = -1) {
