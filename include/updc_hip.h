@@ -337,7 +337,9 @@ int updc_op_c2r(updc_ctx *ctx, const void *spectrum, void *image, int rows, int 
  * its first updc_op_c2r call.  The spectrum's workspace, (FW/2 + 1) * pitch
  * elements of 8 (F32, MIXED) or 16 (F64) bytes, must fit: an FP32 5600 x 5600
  * transform does (125.5 MB), an FP64 one does not (UPDC_ERR_ARG, nothing
- * launched).  Every call on one GPU shares that array, whatever the context:
+ * launched).  An FFT axis too long for one workgroup (four-step) also works
+ * in the scratch updc_prepare reserved, so that path allocates nothing either.
+ * Every call on one GPU shares that array, whatever the context:
  * calls that overlap in time on one GPU (other streams, other threads) race. */
 #define UPDC_NOALLOC_WORKSPACE_BYTES 134217728u
 int updc_op_c2r_noalloc(updc_ctx *ctx, const void *spectrum, void *image, int rows, int cols, int pitch,

@@ -4,8 +4,9 @@ The allocation-free variant runs the same kernels with a static device
 workspace instead of a context buffer, so its images must equal updc_op_c2r's
 bit for bit.  Two calls in a row, the second on a smaller input (a smaller FFT
 grid in another context, and fewer rows and columns), check that the shared
-workspace carries nothing from one call to the next.  A spectrum beyond the
-workspace is refused with UPDC_ERR_ARG before anything runs.
+workspace carries nothing from one call to the next.  FFT shapes with a
+four-step axis run in the scratch updc_prepare reserved.  A spectrum beyond
+the workspace is refused with UPDC_ERR_ARG before anything runs.
 """
 import ctypes as C
 
@@ -73,6 +74,19 @@ def test_noalloc_c2r_equals_op_c2r_twice(small_ds, prec):
     assert np.isfinite(ref).all() and np.array_equal(ref, new)
     big.close()
     small.close()
+
+
+@pytest.mark.parametrize("shape", [(8400, 112), (112, 8400)])
+def test_noalloc_c2r_on_four_step_shapes(small_ds, shape):
+    """An FFT axis too long for one workgroup (four-step) works in the scratch
+    updc_prepare reserved: the allocation-free call runs and equals
+    updc_op_c2r bit for bit."""
+    print(f"case: op_c2r_noalloc vs op_c2r at FFT {shape}", flush=True)
+    c = _prepared(small_ds, "f32", shape)
+    spec, fh, fw = _spectrum(c, 3)
+    ref, new = _both(c, spec, 90, 96, 1.0 / (fh * fw))
+    assert np.isfinite(ref).all() and np.array_equal(ref, new)
+    c.close()
 
 
 def test_noalloc_c2r_refuses_a_spectrum_beyond_the_workspace(small_ds):

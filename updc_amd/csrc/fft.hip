@@ -1540,7 +1540,8 @@ template <class T> hipError_t Fft2<T>::long_space(size_t a, size_t b, size_t cby
     // a captured launch sequence cannot allocate: long_reserve sizes the
     // scratch for the pipeline's calls beforehand
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return hipErrorOutOfMemory;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return hipErrorStreamCaptureUnsupported;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return e;
     auto grow = [](void **p, size_t &have, size_t need, size_t unit) -> hipError_t {

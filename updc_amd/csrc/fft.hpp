@@ -12,6 +12,7 @@
 #pragma once
 #include "kernels.hpp"
 #include <vector>
+#include <algorithm>
 #include <string>
 
 namespace updc {
@@ -81,6 +82,12 @@ template <class T> struct Fft2 {
     // at the full FFT size, allocated now rather than at the first call (a
     // call under graph capture cannot allocate).  No-op when every line fits.
     hipError_t long_reserve(int nexp);
+    // Whether the four-step scratch already covers a row C2R of `rows` rows
+    // and a column pass of one spectrum (true when no axis is four-step).
+    bool long_covers(int rows) const {
+        const size_t r = long_row ? (size_t)rows * s.M : 0, c = long_col ? (size_t)s.NC * s.FH : 0;
+        return std::max(r, c) <= lt_a_n && r <= lt_b_n;
+    }
     size_t spec_elems(int rows) const { return (size_t)s.NC * rp(rows); }
 
     // Row R2C of rows [0, rows) of `batch` images -> column-major spectra

@@ -260,6 +260,8 @@ struct PipeBase {
     virtual void op_c2r_ws(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double scale,
                            void *ws, hipStream_t st) = 0;
     virtual size_t op_c2r_workspace_bytes() const = 0;
+    // op_c2r_ws of `rows` rows would allocate nothing (four-step scratch reserved)
+    virtual bool op_c2r_ws_ready(int rows) const = 0;
     virtual void op_c2r(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double scale,
                         hipStream_t st) = 0;
     virtual void op_gather(Ctx &c, int k, const void *img, void *out, hipStream_t st) = 0;
@@ -1230,6 +1232,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         HIPCHK(fft.template colpass<TA>(COL_FWD, sp, 0, s.FH, sp, 0, s.FH, nullptr, 0, rows, s.FH, 1, st));
     }
     size_t op_c2r_workspace_bytes() const override { return fft.spec_elems(fft.s.FH) * sizeof(Cx<TF>); }
+    bool op_c2r_ws_ready(int rows) const override { return fft.long_covers(rows); }
     void op_c2r_ws(Ctx &c, const void *spec, void *img, int rows, int cols, int pitch, double sc, void *ws,
                    hipStream_t st) override {
         (void)c;
@@ -2146,6 +2149,8 @@ int updc_op_c2r_noalloc(updc_ctx *ctx, const void *spectrum, void *image, int ro
             throw Error(UPDC_ERR_ARG, "spectrum of " + std::to_string(need_bytes) +
                                           " bytes exceeds the allocation-free workspace of " +
                                           std::to_string((size_t)UPDC_NOALLOC_WORKSPACE_BYTES) + " bytes");
+        // a four-step FFT axis works in the scratch updc_prepare reserved
+        need(ctx->pipe->op_c2r_ws_ready(rows), UPDC_ERR_STATE, "four-step FFT scratch not reserved");
         HIPCHK(hipSetDevice(ctx->device));
         void *ws = scratch_on(ctx->device);
         ctx->pipe->op_c2r_ws(*ctx, spectrum, image, rows, cols, pitch, scale, ws, (hipStream_t)stream);
