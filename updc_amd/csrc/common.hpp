@@ -373,3 +373,18 @@ UPDC_HD double dkey_inv(unsigned long long u) {
 // fused multiply-add in the storage type (fmaf for float, fma for double)
 __device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// ---------------------------------------------------------------- global pointers
+// A pointer the kernel reads out of device memory (the ExpDev records) is a
+// generic pointer to the compiler, and loads through it become flat loads:
+// 64-bit VGPR addresses, and every flat load also counts against the LDS
+// counter, so an LDS read after it waits for the memory load as well.  gp()
+// states that the pointer addresses global memory (all of these do).
+// (The host pass of the same source keeps plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using gptr_t = __attribute__((address_space(1))) T *;
+#else
+template <class T> using gptr_t = T *;
+#endif
+template <class T> __device__ __forceinline__ gptr_t<const T> gp(const T *p) { return (gptr_t<const T>)p; }
+template <class T> __device__ __forceinline__ gptr_t<T> gp(T *p) { return (gptr_t<T>)p; }

@@ -495,8 +495,10 @@ template <class TF, class TA> struct Pipe : PipeBase {
             e.c_toff.alloc(ntiles + 1);
             e.c_pb.alloc(ntiles);
             e.c_cnt.alloc(nout);
-            e.c_dp.alloc(std::max<uint64_t>(total, 1));
-            e.c_fr.alloc(std::max<uint64_t>(total, 1));
+            // 16 bytes to spare: the adjoint gather reads the entry runs in
+            // aligned 16-byte pieces
+            e.c_dp.alloc(total + 8);
+            e.c_fr.alloc(total + 4);
             HIPCHK(fill<unsigned int>(err.p, 0u, 1, c.st));
             HIPCHK(csc16_build(e.toff.p, e.tent.p, nout, e.c_toff.p, e.c_pb.p, e.c_cnt.p, e.c_dp.p, e.c_fr.p, err.p,
                                c.st));

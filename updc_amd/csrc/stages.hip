@@ -1148,85 +1148,143 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image(const ExpDev *__restrict_
         }
     }
 }
-// The same gather over the compact CSC (exposures with c16 set).  A wave
-// covers one 64-target tile per target group: the tile's first entry and
-// pixel base are wave-uniform, each lane's first entry is that plus the
-// exclusive prefix sum of the counts of the lanes before it.
+// The same gather over the compact CSC (exposures with c16 set), entry-major.
+// A wave takes one 64-target tile at a time; the tile's first entry and pixel
+// base are wave-uniform, and each lane's first entry is the exclusive prefix
+// sum of the counts of the lanes before it.  The wave's lanes load the tile's
+// entries 64 consecutive ones at a time (so the u16 offsets and f32 fractions
+// arrive in whole lines, with no lane idle on a short target), gather v at
+// each entry's pixel, and leave the (fraction, value) pairs in the wave's LDS
+// window.  Each lane then runs its own target's fmaf chain over its entries
+// there, in entry order, i.e. ascending pixel: the chain of the 8-byte form.
+// LDS accesses of one wave run in issue order, so the window needs no barrier.
+// Software-pipelined: the next tile's entries (and the counts of the one
+// after) are in flight while the current tile's values are gathered.  Every
+// load of the pipelined path is issued by every lane, at a clamped index,
+// and its value selected afterwards: loads under a branch would make the
+// compiler wait for all outstanding loads at the loop head.  The first
+// C16_WIN entries of a tile go through the window at once; a longer tile
+// walks further windows.
+static constexpr int C16_U = 4, C16_WIN = C16_U * 64;
+template <class TF> struct C16Ent { float f; TF v; };
 template <class TF>
 __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restrict__ ex, const TF *__restrict__ v,
                                                          size_t nout, TF *__restrict__ out) {
-    constexpr int G = csci_g<TF>();
+    constexpr int U = C16_U;
+    __shared__ C16Ent<TF> win_all[CSCI_NT / 64][C16_WIN];
     const ExpDev &e = ex[blockIdx.y];
     if (!e.c16) return;
-    const TF *vk = v + e.pix;
-    TF *ok = out + (size_t)blockIdx.y * nout;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const size_t chunk = (size_t)CSCI_NT * G, nch = (nout + chunk - 1) / chunk, per = (nch + 7) / 8;
-    const size_t c_end = min(nch, (size_t)(blockIdx.x % 8 + 1) * per);
-    uint32_t mn[G], mf[G], mp[G];
-    auto meta = [&](size_t cj) {
+    // the record's fields, read once
+    const auto cnt = gp(e.c_cnt);
+    const auto toff = gp(e.c_toff);
+    const auto pbase = gp(e.c_pb);
+    const auto dps = gp(e.c_dp);
+    const auto frs = gp(e.c_fr);
+    const auto vk = gp(v + e.pix);
+    const auto ok = gp(out + (size_t)blockIdx.y * nout);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    C16Ent<TF> *wl = win_all[wave];
+    const size_t ntiles = (nout + 63) / 64, nch = (nout + CSCI_NT - 1) / CSCI_NT, per = (nch + 7) / 8;
+    const size_t c_end = min(nch, (size_t)(blockIdx.x % 8 + 1) * per), step = gridDim.x / 8;
+    // Raw loads of a chunk ahead: the lane's count and the tile's first entry
+    // and base.  Values are selected where they are used, one iteration after
+    // their loads: a select next to its load would make the loop wait for it.
+    uint32_t m_cnt, m_fst, m_pb;
+    auto meta = [&](size_t cj) {  // (a chunk past the end reads the last one)
+        cj = min(cj, nch - 1);
+        const size_t tb = cj * CSCI_NT + (size_t)wave * 64, t = tb + lane, tile = min(tb >> 6, ntiles - 1);
+        m_cnt = cnt[min(t, nout - 1)];
+        m_fst = toff[tile];  // a tile past the end (tb >= nout) has no targets: its counts are zero
+        m_pb = pbase[tile];
+    };
+    // counts of chunk cj (loaded by meta) -> lane offsets (wave prefix sum) ->
+    // raw loads of the tile's first C16_WIN entries (entries past the tile's
+    // last read its last one, or for an empty tile the allocation's spare tail)
+    uint32_t c_n, c_s, c_fst, c_pb, c_tot, c_dp[U];
+    float c_fr[U];
+    auto take = [&](size_t cj, uint32_t &n, uint32_t &s, uint32_t &fst, uint32_t &pb, uint32_t &tot,
+                    uint32_t (&dp)[U], float (&fr)[U]) {
+        cj = min(cj, nch - 1);
+        const size_t t = cj * CSCI_NT + (size_t)wave * 64 + lane;
+        n = t < nout ? m_cnt : 0u;
+        uint32_t x = n;
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const size_t tb = cj * chunk + (size_t)g * CSCI_NT + (size_t)wave * 64;
-            const size_t t = tb + lane;
-            const bool live = tb < nout;
-            mn[g] = t < nout ? (uint32_t)e.c_cnt[t] : 0u;
-            mf[g] = live ? e.c_toff[tb >> 6] : 0u;
-            mp[g] = live ? e.c_pb[tb >> 6] : 0u;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+            if (lane >= o) x += y;
+        }
+        s = x - n;
+        tot = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+        fst = m_fst;
+        pb = m_pb;
+        const uint32_t last = tot ? tot - 1 : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t jc = fst + min((uint32_t)(u * 64 + lane), last);
+            dp[u] = dps[jc];
+            fr[u] = frs[jc];
         }
     };
-    size_t ci0 = (size_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (ci0 < c_end) meta(ci0);
-    for (size_t ci = ci0; ci < c_end; ci += gridDim.x / 8) {
-        uint32_t a[G], n[G], pb[G], fst[G];
+    size_t ci = (size_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (ci >= c_end) return;
+    meta(ci);
+    take(ci, c_n, c_s, c_fst, c_pb, c_tot, c_dp, c_fr);
+    meta(ci + step);
+    for (; ci < c_end; ci += step) {
+        // fractions of the current tile into the window (every slot is written;
+        // slots past the tile's entries are never read)
 #pragma unroll
-        for (int g = 0; g < G; ++g) { n[g] = mn[g]; fst[g] = mf[g]; pb[g] = mp[g]; }
-        if (ci + gridDim.x / 8 < c_end) meta(ci + gridDim.x / 8);
+        for (int u = 0; u < U; ++u) wl[u * 64 + lane].f = c_fr[u];
+        // values (an entry past the tile reads v at the tile's base)
+        TF wv[U];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const uint32_t first = fst[g];
-            uint32_t x = n[g];
+        for (int u = 0; u < U; ++u) wv[u] = vk[c_pb + ((uint32_t)(u * 64 + lane) < c_tot ? c_dp[u] : 0u)];
+        // the next tiles' loads, unconditionally (past the end they are unused):
+        // the values' wait below leaves them in flight
+        uint32_t n_n, n_s, n_fst, n_pb, n_tot, n_dp[U];
+        float n_fr[U];
+        take(ci + step, n_n, n_s, n_fst, n_pb, n_tot, n_dp, n_fr);
+        meta(ci + 2 * step);
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = (uint32_t)__shfl_up((int)x, o);
-                if (lane >= o) x += y;
-            }
-            a[g] = first + x - n[g];
+        for (int u = 0; u < U; ++u) wl[u * 64 + lane].v = wv[u];
+        const uint32_t j0 = c_s, j1 = c_s + c_n;
+        TF acc = (TF)0;
+        for (uint32_t j = j0; j < min(j1, (uint32_t)C16_WIN); ++j) {
+            const C16Ent<TF> q = wl[j];
+            acc = fmat((TF)q.f, q.v, acc);
         }
-        uint32_t dp[G][CSCI_U];
-        float fr[G][CSCI_U];
-        TF wv[G][CSCI_U];
+        // tiles beyond one window (wave-uniform)
+        for (uint32_t wb = C16_WIN; wb < c_tot; wb += C16_WIN) {
 #pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int j = 0; j < CSCI_U; ++j) {
-                dp[g][j] = (uint32_t)j < n[g] ? (uint32_t)e.c_dp[a[g] + j] : 0u;
-                fr[g][j] = (uint32_t)j < n[g] ? e.c_fr[a[g] + j] : 0.0f;
+            for (int u = 0; u < U; ++u) {
+                const uint32_t j = wb + (uint32_t)(u * 64 + lane);
+                if (j < c_tot) wl[u * 64 + lane] = C16Ent<TF>{frs[c_fst + j], vk[c_pb + (uint32_t)dps[c_fst + j]]};
             }
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int j = 0; j < CSCI_U; ++j) wv[g][j] = (uint32_t)j < n[g] ? vk[pb[g] + dp[g][j]] : (TF)0;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const size_t t = ci * chunk + (size_t)g * CSCI_NT + threadIdx.x;
-            TF acc = (TF)0;
-#pragma unroll
-            for (int j = 0; j < CSCI_U; ++j)
-                if ((uint32_t)j < n[g]) acc = fmat((TF)fr[g][j], wv[g][j], acc);
-            for (uint32_t j = CSCI_U; j < n[g]; ++j)
-                acc = fmat((TF)e.c_fr[a[g] + j], vk[pb[g] + (uint32_t)e.c_dp[a[g] + j]], acc);
-            if (t < nout) ok[t] = acc;
+            for (uint32_t j = max(j0, wb); j < min(j1, wb + (uint32_t)C16_WIN); ++j) {
+                const C16Ent<TF> q = wl[j - wb];
+                acc = fmat((TF)q.f, q.v, acc);
+            }
         }
+        const size_t t = ci * CSCI_NT + threadIdx.x;
+        if (t < nout) ok[t] = acc;
+        c_n = n_n; c_s = n_s; c_fst = n_fst; c_pb = n_pb; c_tot = n_tot;
+#pragma unroll
+        for (int u = 0; u < U; ++u) { c_dp[u] = n_dp[u]; c_fr[u] = n_fr[u]; }
     }
 }
 template <class TF>
 hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *out, bool any_c16, bool any_wide,
                      hipStream_t st) {
     if (nexp <= 0) return hipSuccess;
-    const size_t chunk = (size_t)CSCI_NT * csci_g<TF>();
-    const unsigned gx = (unsigned)((std::min<size_t>((nout + chunk - 1) / chunk, (size_t)std::max(1, 16384 / nexp)) + 7) / 8 * 8);
-    if (any_c16) hipLaunchKernelGGL((k_csc_image16<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
+    // grid-stride over chunks of CSCI_NT * G targets, at most 16384 blocks in all,
+    // a multiple of 8 per exposure (the kernels' XCD ranges)
+    auto grid = [&](int g) {
+        const size_t chunk = (size_t)CSCI_NT * g;
+        return (unsigned)((std::min<size_t>((nout + chunk - 1) / chunk, (size_t)std::max(1, 16384 / nexp)) + 7) / 8 * 8);
+    };
+    const unsigned gx = grid(csci_g<TF>());
+    if (any_c16)
+        hipLaunchKernelGGL((k_csc_image16<TF>), dim3(grid(1), nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
     if (any_wide)
         hipLaunchKernelGGL((k_csc_image<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out, (int)any_c16);
     return hipGetLastError();
