@@ -387,4 +387,14 @@ template <class T> using gptr_t = __attribute__((address_space(1))) T *;
 template <class T> using gptr_t = T *;
 #endif
 template <class T> __device__ __forceinline__ gptr_t<const T> gp(const T *p) { return (gptr_t<const T>)p; }
+// Read-only for the whole launch, and read at wave-uniform addresses: the
+// constant address space lets the compiler use scalar loads that it waits for
+// where the value is used (a vector load of a uniform value is moved to a
+// scalar register right away, which waits for every load issued before it).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using cptr_t = __attribute__((address_space(4))) T *;
+#else
+template <class T> using cptr_t = T *;
+#endif
+template <class T> __device__ __forceinline__ cptr_t<const T> cp(const T *p) { return (cptr_t<const T>)p; }
 template <class T> __device__ __forceinline__ gptr_t<T> gp(T *p) { return (gptr_t<T>)p; }

@@ -633,8 +633,13 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
     const bool d16 = e.encoding == UPDC_LM_DELTA16;
     const uintptr_t lbase = d16 ? (uintptr_t)e.d16 : (uintptr_t)e.t32;
     const uint32_t lsz = d16 ? 2u : 4u;
-    const uint32_t *__restrict__ off = e.off;
-    const float *__restrict__ frac = e.frac;
+    // the record's arrays as global pointers (loads through generic pointers
+    // would be flat loads, which also hold up every LDS read behind them)
+    const auto off = gp(e.off);
+    const auto frac = gp(e.frac);
+    const auto ebase = gp(e.base);
+    const auto ed16 = gp(e.d16);
+    const auto et32 = gp(e.t32);
     const TF *bl = blur + (size_t)k * blur_bstride;
     const uint16_t *s16 = (const uint16_t *)s_lv;
     const uint32_t *s32 = (const uint32_t *)s_lv;
@@ -647,10 +652,11 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
     // Software pipeline over the block's chunks: while chunk i's blur gathers
     // run, chunk i+1's staging blocks and pixel operands are in flight, and
     // chunk i+2's link range is being read.
+    const auto offc = cp(e.off);
     auto range = [&](uint32_t q0, uint32_t &A0, uint32_t &A1) {
         if (q0 < nin) {
-            A0 = off[q0];
-            A1 = off[min(q0 + (uint32_t)GATHER_NT, nin)];
+            A0 = offc[q0];
+            A1 = offc[min(q0 + (uint32_t)GATHER_NT, nin)];
         }
     };
     uint4 lv[GATHER_SLOTS], fv[GATHER_SLOTS];
@@ -675,23 +681,20 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
 #pragma unroll
         for (int s = 0; s < GATHER_SLOTS; ++s) {
             const uint32_t i = threadIdx.x + s * GATHER_NT;
-            lv[s] = i < n_nl ? ((const uint4 *)la0)[i] : uint4{};
-            fv[s] = i < n_nf ? ((const uint4 *)fa0)[i] : uint4{};
+            lv[s] = fv[s] = uint4{};
+            if (i < n_nl) lv[s] = ((gptr_t<const uint4>)la0)[i];
+            if (i < n_nf) fv[s] = ((gptr_t<const uint4>)fa0)[i];
         }
-        const uint32_t q = q0 + threadIdx.x;
-        const uint32_t qend = min(q0 + (uint32_t)GATHER_NT, nin);
-        n_z0 = n_z1 = n_b0 = 0;
-        n_wp = n_sv = (TF)0;
-        n_fz = 0;
-        if (q < qend) {
-            const size_t gq = e.pix + q;
-            n_z0 = off[q];
-            n_z1 = off[q + 1];
-            if (d16) n_b0 = e.base[q];
-            n_wp = w[gq];
-            n_sv = sci[gq];
-            n_fz = frozen[gq];
-        }
+        // the lane's pixel operands, issued by every lane (a lane past the
+        // exposure's end reads its last pixel and never uses the values)
+        const uint32_t q = min(q0 + threadIdx.x, nin - 1);
+        const size_t gq = e.pix + q;
+        n_z0 = off[q];
+        n_z1 = off[q + 1];
+        n_b0 = d16 ? ebase[q] : 0u;
+        n_wp = w[gq];
+        n_sv = sci[gq];
+        n_fz = frozen[gq];
     };
     uint32_t Zn0 = 0, Zn1 = 0;
     if (p0 < nin) {
@@ -744,9 +747,9 @@ __global__ __launch_bounds__(GATHER_NT, (gather_waves<TF>())) void k_gather(cons
                         pred = fmat((TF)sfr[f0 + j], bl[t], pred);
                     }
                 } else if (d16) {
-                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)frac[z], bl[b0 + (uint32_t)e.d16[z]], pred);
+                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)frac[z], bl[b0 + (uint32_t)ed16[z]], pred);
                 } else {
-                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)frac[z], bl[e.t32[z]], pred);
+                    for (uint32_t z = z0; z < z1; ++z) pred = fmat((TF)frac[z], bl[et32[z]], pred);
                 }
                 TA raw;
                 if constexpr (sizeof(TA) == 4) raw = pred + e.mean_offset_f;
