@@ -247,6 +247,70 @@ __device__ __forceinline__ void fft_lds_range(Cx<T> *buf, const FftLen &p, int s
     }
 }
 
+// The same stage out of place: reads `in`, writes `out` (two LDS buffers), so
+// one barrier (after the writes) separates it from the next stage instead of
+// two.  Single line (lg = 0).
+template <int R, bool INV, class T, int NT, int VMAX>
+__device__ __forceinline__ void fft_stage_pp(const Cx<T> *__restrict__ in, Cx<T> *__restrict__ out, int L, int Ns,
+                                             unsigned int mg, const Cx<T> *__restrict__ tws) {
+    constexpr int NB = VMAX / R;
+    static_assert(NB >= 1, "VMAX too small for radix");
+    int LR = L / R;
+    asm volatile("" : "+s"(LR));
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        const int i = tid + r * NT;
+        if (i < LR) {
+            Cx<T> v[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = in[i + q * LR];
+            const int hi = Ns == 1 ? i : (int)__umulhi((unsigned int)i, mg);
+            const int k = i - hi * Ns;
+            if (Ns > 1) {
+                const Cx<T> *wk = tws + k * (R - 1);
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    Cx<T> w = wk[q - 1];
+                    if (INV) w.im = -w.im;
+                    v[q] = cmul(v[q], w);
+                }
+            }
+            bfly<R, INV>(v);
+            const int j = hi * Ns * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) out[j + q * Ns] = v[q];
+        }
+    }
+    __syncthreads();
+}
+// Stages [s0, s1) out of place, alternating between the buffers *cur and
+// *oth; on return *cur holds the result.  twb: the plan's stage twiddle
+// tables (p.tw or a copy of them, e.g. in LDS).
+template <bool INV, class T, int NT, int VMAX>
+__device__ __forceinline__ void fft_lds_range_pp(Cx<T> *&cur, Cx<T> *&oth, const FftLen &p, const Cx<T> *twb,
+                                                 int s0, int s1) {
+    for (int s = s0; s < s1; ++s) {
+        const int R = p.rad[s], Ns = p.ns[s];
+        const unsigned int mg = p.mg[s];
+        const Cx<T> *tw = twb + p.to[s];
+        switch (R) {
+            case 2: fft_stage_pp<2, INV, T, NT, VMAX>(cur, oth, p.L, Ns, mg, tw); break;
+            case 3: fft_stage_pp<3, INV, T, NT, VMAX>(cur, oth, p.L, Ns, mg, tw); break;
+            case 4: fft_stage_pp<4, INV, T, NT, VMAX>(cur, oth, p.L, Ns, mg, tw); break;
+            case 5: fft_stage_pp<5, INV, T, NT, VMAX>(cur, oth, p.L, Ns, mg, tw); break;
+            case 7: fft_stage_pp<7, INV, T, NT, VMAX>(cur, oth, p.L, Ns, mg, tw); break;
+            case 16:
+                if constexpr (VMAX >= 16) fft_stage_pp<16, INV, T, NT, VMAX>(cur, oth, p.L, Ns, mg, tw);
+                break;
+            default: fft_stage_pp<8, INV, T, NT, VMAX>(cur, oth, p.L, Ns, mg, tw); break;
+        }
+        Cx<T> *t = cur;
+        cur = oth;
+        oth = t;
+    }
+}
+
 // Register-resident edge stages of a transform whose first and last radix
 // are both RE (LR = L / RE).  Thread tid owns butterflies i_r = tid + r NT
 // (i_r < LR) and the RE values of each at element positions i_r + q LR, the
@@ -254,7 +318,7 @@ __device__ __forceinline__ void fft_lds_range(Cx<T> *buf, const FftLen &p, int s
 //   edge_first: stage 0 (Ns = 1, no twiddles) from registers into LDS
 //               (outputs at i RE + q), then a barrier.
 //   edge_last : the last stage (Ns = LR, k = i) from LDS back into registers,
-//               then a barrier so the buffer may be rewritten.
+//               then (SYNC) a barrier so the buffer may be rewritten.
 template <int RE, bool INV, class T, int NT, int NBE>
 __device__ __forceinline__ void edge_first(Cx<T> (&rb)[NBE][RE], Cx<T> *buf, int LR) {
     const int tid = threadIdx.x;
@@ -272,7 +336,7 @@ __device__ __forceinline__ void edge_first(Cx<T> (&rb)[NBE][RE], Cx<T> *buf, int
     }
     __syncthreads();
 }
-template <int RE, bool INV, class T, int NT, int NBE>
+template <int RE, bool INV, class T, int NT, int NBE, bool SYNC = true>
 __device__ __forceinline__ void edge_last(Cx<T> (&rb)[NBE][RE], const Cx<T> *buf, int LR,
                                           const Cx<T> *__restrict__ tws) {
     const int tid = threadIdx.x;
@@ -299,7 +363,7 @@ __device__ __forceinline__ void edge_last(Cx<T> (&rb)[NBE][RE], const Cx<T> *buf
             for (int q = 0; q < RE; ++q) rb[r][q] = cx<T>((T)0, (T)0);
         }
     }
-    __syncthreads();
+    if (SYNC) __syncthreads();
 }
 
 // Full transform over the LDS buffer (natural order in, natural order out):

@@ -650,7 +650,14 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
 // registers before the transform that precedes its use.
 // EV: complex values per thread per register set (the edge butterflies of
 // one thread, EV / RE of them).
-template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF, int EV>
+// PP: the LDS holds two transform buffers (2 FH values).  The middle stages
+// then run out of place, one barrier each instead of two, and the last edge
+// stage reads one buffer while the next transform's first edge stage writes
+// the other, so it needs no barrier of its own.  (The stage twiddles stay in
+// global memory: read from LDS, with a butterfly's R-1 twiddles adjacent,
+// they add bank-conflicted LDS traffic to passes that LDS bounds already, and
+// the broadcast and accumulate passes slowed by 10 %.)
+template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF, int EV, bool PP>
 __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__restrict__ in, size_t in_bs, int in_rp,
                                               Cx<T> *__restrict__ out, size_t out_bs, int out_rp,
                                               const Cx<T> *__restrict__ psf, size_t psf_bs, int psf_rp, int hin,
@@ -663,6 +670,9 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
     const int LR = f.L / RE, ns = f.nst, tid = threadIdx.x;
     const Cx<T> *twl = (const Cx<T> *)f.tw + f.to[ns - 1];
     Cx<T> rb[NBE][RE];
+    // ping-pong state: the buffer the next first edge stage writes
+    Cx<T> *bx = buf, *by = buf + f.L;
+    const Cx<T> *twb = (const Cx<T> *)f.tw;
 
     auto load = [&](Cx<T>(&d)[NBE][RE], const Cx<T> *c, int h) {
 #pragma unroll
@@ -683,14 +693,32 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
             }
     };
     auto fwd = [&]() {
-        edge_first<RE, false, T, NT, NBE>(rb, buf, LR);
-        fft_lds_range<false, T, NT, VMAX>(buf, f, 1, ns - 1);
-        edge_last<RE, false, T, NT, NBE>(rb, buf, LR, twl);
+        if constexpr (PP) {
+            edge_first<RE, false, T, NT, NBE>(rb, bx, LR);
+            fft_lds_range_pp<false, T, NT, VMAX>(bx, by, f, twb, 1, ns - 1);
+            edge_last<RE, false, T, NT, NBE, false>(rb, bx, LR, twl);
+            Cx<T> *t = bx;  // the next first edge stage writes the other buffer
+            bx = by;
+            by = t;
+        } else {
+            edge_first<RE, false, T, NT, NBE>(rb, buf, LR);
+            fft_lds_range<false, T, NT, VMAX>(buf, f, 1, ns - 1);
+            edge_last<RE, false, T, NT, NBE>(rb, buf, LR, twl);
+        }
     };
     auto inv = [&]() {
-        edge_first<RE, true, T, NT, NBE>(rb, buf, LR);
-        fft_lds_range<true, T, NT, VMAX>(buf, f, 1, ns - 1);
-        edge_last<RE, true, T, NT, NBE>(rb, buf, LR, twl);
+        if constexpr (PP) {
+            edge_first<RE, true, T, NT, NBE>(rb, bx, LR);
+            fft_lds_range_pp<true, T, NT, VMAX>(bx, by, f, twb, 1, ns - 1);
+            edge_last<RE, true, T, NT, NBE, false>(rb, bx, LR, twl);
+            Cx<T> *t = bx;
+            bx = by;
+            by = t;
+        } else {
+            edge_first<RE, true, T, NT, NBE>(rb, buf, LR);
+            fft_lds_range<true, T, NT, VMAX>(buf, f, 1, ns - 1);
+            edge_last<RE, true, T, NT, NBE>(rb, buf, LR, twl);
+        }
     };
 
     if (MODE == COL_FWD || MODE == COL_INV) {
@@ -749,7 +777,9 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
             if (fa.L > 0) {
                 // the inverse of the sum in TA (plan fa: the same radices, TA
                 // twiddles) over the whole LDS allocation, FH TA values; the
-                // last forward transform's edge stage ended with a barrier
+                // last forward transform's edge stage ended with a barrier,
+                // except in ping-pong mode
+                if (PP) __syncthreads();
                 Cx<TA> *bufa = reinterpret_cast<Cx<TA> *>(smem_raw);
                 const Cx<TA> *twla = (const Cx<TA> *)fa.tw + fa.to[ns - 1];
                 edge_first<RE, true, TA, NT, NBE>(acc, bufa, LR);
@@ -1160,27 +1190,32 @@ template <class T, class TA, int MODE> static void *col_fn() {
     return (void *)k_col<T, TA, ColCfg<T>::NT, ColCfg<T>::VMAX, ColCfg<T>::XMAX, MODE>;
 }
 
-template <class T, class TA, int MODE, bool PF> static void *col_e_fn_pf(int re) {
+template <class T, class TA, int MODE, bool PF, bool PP> static void *col_e_fn_pf(int re) {
     constexpr int NT = ColCfg<T>::NT, V = ColCfg<T>::VMAX, EV = ColCfg<T>::EV;
     switch (re) {
-        case 3: return (void *)k_col_e<T, TA, NT, V, 3, MODE, PF, EV>;
-        case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF, EV>;
-        case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF, EV>;
-        case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF, EV>;
-        case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF, EV>;
+        case 3: return (void *)k_col_e<T, TA, NT, V, 3, MODE, PF, EV, PP>;
+        case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF, EV, PP>;
+        case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF, EV, PP>;
+        case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF, EV, PP>;
+        case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF, EV, PP>;
         default: return nullptr;
     }
 }
-template <class T, class TA, int MODE> static void *col_e_fn(int re) {
+// pp: the two-buffer kernel (FP32 only; FP64 columns fill the LDS with one)
+template <class T, class TA, int MODE> static void *col_e_fn(int re, bool pp) {
     constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ || MODE == COL_ACC);
-    return col_e_fn_pf<T, TA, MODE, PF>(re);
+    if constexpr (sizeof(T) == 4)
+        if (pp) return col_e_fn_pf<T, TA, MODE, PF, true>(re);
+    return col_e_fn_pf<T, TA, MODE, PF, false>(re);
 }
 
 // COL_ACC kernels exist only for a sum wider than the transform type.
-template <class T, class TA> static void *col_acc_e_fn(int re) {
-    if constexpr (sizeof(TA) > sizeof(T)) return col_e_fn<T, TA, COL_ACC>(re);
+template <class T, class TA> static void *col_acc_e_fn(int re, bool pp) {
+    if constexpr (sizeof(TA) > sizeof(T)) return col_e_fn<T, TA, COL_ACC>(re, pp);
     return nullptr;
 }
+// LDS of the two-buffer column kernel: two lines
+template <class T> static size_t col_pp_lds(int FH, const FftLen &) { return 2 * (size_t)FH * sizeof(Cx<T>); }
 
 // ------------------------------------------------------------------ four-step host side
 // 256 threads, 16 complex values per thread and stage; B = 16 (FP32) or 8
@@ -1290,7 +1325,10 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     post = upload_roots<T>(FW, s.M + 1, -1, err);
     if (!post) return err;
     const int rlds = (int)row_lds<T>(s.M), rlds_p = (int)row_lds_p<T>(s.M, row_ntw);
-    const int col_lds = (int)((size_t)FH * sizeof(Cx<T>));
+    // two transform buffers where they fit the LDS budget (the FP32 edge
+    // kernel's out-of-place stages)
+    col_pp = allow_pp && col_re && sizeof(T) == 4 && col_pp_lds<T>(FH, col) <= LDS_BUDGET;
+    const int col_lds = (int)(col_pp ? col_pp_lds<T>(FH, col) : (size_t)FH * sizeof(Cx<T>));
     void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_r2c_p_fn<T>()};
     void *fns_row_p[] = {row_c2r_p_fn<T>(), row_c2r_upd_fn<T, float>(), row_c2r_upd_fn<T, T>()};
     // persistent row passes: one workgroup per CU, prefetch registers must cover a tile
@@ -1319,16 +1357,16 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
             return "cannot raise LDS limit for the column pass";
     if (col_re) {
-        void *fns_e[] = {col_e_fn<T, T, COL_FWD>(col_re), col_e_fn<T, T, COL_INV>(col_re),
-                         col_e_fn<T, T, COL_BCAST>(col_re), col_e_fn<T, T, COL_ADJ>(col_re),
-                         col_e_fn<T, double, COL_ADJ>(col_re)};
+        void *fns_e[] = {col_e_fn<T, T, COL_FWD>(col_re, col_pp), col_e_fn<T, T, COL_INV>(col_re, col_pp),
+                         col_e_fn<T, T, COL_BCAST>(col_re, col_pp), col_e_fn<T, T, COL_ADJ>(col_re, col_pp),
+                         col_e_fn<T, double, COL_ADJ>(col_re, col_pp)};
         for (void *fn : fns_e)
             if (!fn || hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
                 return "cannot raise LDS limit for the column pass";
     }
     if constexpr (sizeof(T) < sizeof(double)) {
         // the accumulate pass may run its FP64 inverse over FH double values
-        void *fn_acc = col_re ? col_acc_e_fn<T, double>(col_re) : col_fn<T, double, COL_ACC>();
+        void *fn_acc = col_re ? col_acc_e_fn<T, double>(col_re, col_pp) : col_fn<T, double, COL_ACC>();
         const int acc_lds = (int)std::min<size_t>((size_t)FH * sizeof(Cx<double>), LDS_BUDGET);
         if (!fn_acc || hipFuncSetAttribute(fn_acc, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            std::max(col_lds, acc_lds)) != hipSuccess)
@@ -1438,11 +1476,11 @@ hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, i
     hout = std::min(hout, s.FH);
     const int irp = rp(in_rows), orp = rp(out_rows), prp = rp(s.FH);
     if (col_re) {
-        void *fn = mode == COL_FWD     ? col_e_fn<T, T, COL_FWD>(col_re)
-                   : mode == COL_INV   ? col_e_fn<T, T, COL_INV>(col_re)
-                   : mode == COL_BCAST ? col_e_fn<T, T, COL_BCAST>(col_re)
-                   : mode == COL_ADJ   ? col_e_fn<T, TA, COL_ADJ>(col_re)
-                                       : col_acc_e_fn<T, TA>(col_re);
+        void *fn = mode == COL_FWD     ? col_e_fn<T, T, COL_FWD>(col_re, col_pp)
+                   : mode == COL_INV   ? col_e_fn<T, T, COL_INV>(col_re, col_pp)
+                   : mode == COL_BCAST ? col_e_fn<T, T, COL_BCAST>(col_re, col_pp)
+                   : mode == COL_ADJ   ? col_e_fn<T, TA, COL_ADJ>(col_re, col_pp)
+                                       : col_acc_e_fn<T, TA>(col_re, col_pp);
         if (!fn) return hipErrorInvalidValue;
         const int nc = s.NC, nexp = batch;
         const Cx<T> *pin = in, *ppsf = psf;
@@ -1451,6 +1489,7 @@ hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, i
                         (void *)&pout, (void *)&out_bstride, (void *)&orp, (void *)&ppsf, (void *)&psf_bstride,
                         (void *)&prp, (void *)&hin, (void *)&hout, (void *)&nexp, (void *)&fa};
         const dim3 grid(gx, (mode == COL_FWD || mode == COL_INV) ? batch : 1);
+        if (col_pp) lds = std::max(lds, col_pp_lds<T>(s.FH, col));
         return hipLaunchKernel(fn, grid, dim3(CNT), args, lds, st);
     }
     switch (mode) {

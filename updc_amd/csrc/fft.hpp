@@ -61,6 +61,8 @@ template <class T> struct Fft2 {
     FftLen row{}, col{};
     Cx<T> *tw_row = nullptr, *tw_col = nullptr, *post = nullptr;  // device tables
     int col_re = 0;          // edge radix of the column plan (0: LDS-only column kernel)
+    bool col_pp = false;     // edge column kernel with two LDS transform buffers (out-of-place stages)
+    bool allow_pp = true;    // set before init(): false keeps the edge column kernel in place
     bool allow_edge = true;  // set before init(): false forces the LDS-only column kernel
     bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit
     bool row_odd_first = true;   // set before init(): an odd radix first in the row plan (LDS banks)
