@@ -462,3 +462,18 @@ template <class T> using cptr_t = T *;
 #endif
 template <class T> __device__ __forceinline__ cptr_t<const T> cp(const T *p) { return (cptr_t<const T>)p; }
 template <class T> __device__ __forceinline__ gptr_t<T> gp(T *p) { return (gptr_t<T>)p; }
+
+// ---------------------------------------------------------------- wave scan
+// Inclusive prefix sum over the 64 lanes of a wave with DPP row shifts and the
+// row broadcasts of the GFX9 family (no LDS round trip).  Every lane of the
+// wave must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    int v = (int)x;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return (uint32_t)v;
+}

@@ -1188,6 +1188,9 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restric
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     C16Ent<TF> *wl = win_all[wave];
     const size_t ntiles = (nout + 63) / 64, nch = (nout + CSCI_NT - 1) / CSCI_NT, per = (nch + 7) / 8;
+    // the last entry of the map and the last pixel of the exposure: bounds of
+    // every clamped read (an empty map reads the allocation's spare slot 0)
+    const uint32_t nent = toff[ntiles], ent_last = nent ? nent - 1 : 0u, pix_last = e.nin ? e.nin - 1 : 0u;
     const size_t c_end = min(nch, (size_t)(blockIdx.x % 8 + 1) * per), step = gridDim.x / 8;
     // Raw loads of a chunk ahead: the lane's count and the tile's first entry
     // and base.  Values are selected where they are used, one iteration after
@@ -1210,12 +1213,7 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restric
         cj = min(cj, nch - 1);
         const size_t t = cj * CSCI_NT + (size_t)wave * 64 + lane;
         n = t < nout ? m_cnt : 0u;
-        uint32_t x = n;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)x, o);
-            if (lane >= o) x += y;
-        }
+        const uint32_t x = wave_incl_scan(n);
         s = x - n;
         tot = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
         fst = m_fst;
@@ -1223,7 +1221,9 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restric
         const uint32_t last = tot ? tot - 1 : 0u;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t jc = fst + min((uint32_t)(u * 64 + lane), last);
+            // (clamped to the map's entries as well: a tile's range never
+            // leaves them, and a read never leaves the allocation)
+            const uint32_t jc = min(fst + min((uint32_t)(u * 64 + lane), last), ent_last);
             dp[u] = dps[jc];
             fr[u] = frs[jc];
         }
@@ -1241,7 +1241,8 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restric
         // values (an entry past the tile reads v at the tile's base)
         TF wv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) wv[u] = vk[c_pb + ((uint32_t)(u * 64 + lane) < c_tot ? c_dp[u] : 0u)];
+        for (int u = 0; u < U; ++u)
+            wv[u] = vk[min(c_pb + ((uint32_t)(u * 64 + lane) < c_tot ? c_dp[u] : 0u), pix_last)];
         // the next tiles' loads, unconditionally (past the end they are unused):
         // the values' wait below leaves them in flight
         uint32_t n_n, n_s, n_fst, n_pb, n_tot, n_dp[U];
