@@ -1167,14 +1167,17 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image(const ExpDev *__restrict_
 // and its value selected afterwards: loads under a branch would make the
 // compiler wait for all outstanding loads at the loop head.  The first
 // C16_WIN entries of a tile go through the window at once; a longer tile
-// walks further windows.
-static constexpr int C16_U = 4, C16_WIN = C16_U * 64;
+// walks further windows.  C16_G tiles per wave and iteration.  With 8
+// exposures of 2048² on a 4096² grid (about 144 entries per tile): 0.56 ms
+// with a 192-entry window (three rounds of loads, every one issued), within
+// 2 % for one to four tiles per iteration; 0.59-0.60 ms with 256 entries.
+static constexpr int C16_U = 3, C16_WIN = C16_U * 64, C16_G = 2;
 template <class TF> struct C16Ent { float f; TF v; };
 template <class TF>
 __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restrict__ ex, const TF *__restrict__ v,
                                                          size_t nout, TF *__restrict__ out) {
-    constexpr int U = C16_U;
-    __shared__ C16Ent<TF> win_all[CSCI_NT / 64][C16_WIN];
+    constexpr int U = C16_U, G = C16_G;
+    __shared__ C16Ent<TF> win_all[CSCI_NT / 64][G][C16_WIN];
     const ExpDev &e = ex[blockIdx.y];
     if (!e.c16) return;
     // the record's fields, read once
@@ -1186,8 +1189,9 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restric
     const auto vk = gp(v + e.pix);
     const auto ok = gp(out + (size_t)blockIdx.y * nout);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    C16Ent<TF> *wl = win_all[wave];
-    const size_t ntiles = (nout + 63) / 64, nch = (nout + CSCI_NT - 1) / CSCI_NT, per = (nch + 7) / 8;
+    // chunks of G * CSCI_NT targets; wave w takes tiles g * CSCI_NT / 64 + w of a chunk
+    constexpr size_t CH = (size_t)G * CSCI_NT;
+    const size_t ntiles = (nout + 63) / 64, nch = (nout + CH - 1) / CH, per = (nch + 7) / 8;
     // the last entry of the map and the last pixel of the exposure: bounds of
     // every clamped read (an empty map reads the allocation's spare slot 0)
     const uint32_t nent = toff[ntiles], ent_last = nent ? nent - 1 : 0u, pix_last = e.nin ? e.nin - 1 : 0u;
@@ -1195,85 +1199,99 @@ __global__ __launch_bounds__(CSCI_NT) void k_csc_image16(const ExpDev *__restric
     // Raw loads of a chunk ahead: the lane's count and the tile's first entry
     // and base.  Values are selected where they are used, one iteration after
     // their loads: a select next to its load would make the loop wait for it.
-    uint32_t m_cnt, m_fst, m_pb;
+    uint32_t m_cnt[G], m_fst[G], m_pb[G];
     auto meta = [&](size_t cj) {  // (a chunk past the end reads the last one)
         cj = min(cj, nch - 1);
-        const size_t tb = cj * CSCI_NT + (size_t)wave * 64, t = tb + lane, tile = min(tb >> 6, ntiles - 1);
-        m_cnt = cnt[min(t, nout - 1)];
-        m_fst = toff[tile];  // a tile past the end (tb >= nout) has no targets: its counts are zero
-        m_pb = pbase[tile];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const size_t tb = cj * CH + (size_t)g * CSCI_NT + (size_t)wave * 64, t = tb + lane;
+            const size_t tile = min(tb >> 6, ntiles - 1);
+            m_cnt[g] = cnt[min(t, nout - 1)];
+            m_fst[g] = toff[tile];  // a tile past the end (tb >= nout) has no targets: its counts are zero
+            m_pb[g] = pbase[tile];
+        }
     };
     // counts of chunk cj (loaded by meta) -> lane offsets (wave prefix sum) ->
-    // raw loads of the tile's first C16_WIN entries (entries past the tile's
+    // raw loads of each tile's first C16_WIN entries (entries past the tile's
     // last read its last one, or for an empty tile the allocation's spare tail)
-    uint32_t c_n, c_s, c_fst, c_pb, c_tot, c_dp[U];
-    float c_fr[U];
-    auto take = [&](size_t cj, uint32_t &n, uint32_t &s, uint32_t &fst, uint32_t &pb, uint32_t &tot,
-                    uint32_t (&dp)[U], float (&fr)[U]) {
+    struct St {
+        uint32_t n[G], s[G], fst[G], pb[G], tot[G], dp[G][U];
+        float fr[G][U];
+    };
+    auto take = [&](size_t cj, St &a) __attribute__((always_inline)) {
         cj = min(cj, nch - 1);
-        const size_t t = cj * CSCI_NT + (size_t)wave * 64 + lane;
-        n = t < nout ? m_cnt : 0u;
-        const uint32_t x = wave_incl_scan(n);
-        s = x - n;
-        tot = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
-        fst = m_fst;
-        pb = m_pb;
-        const uint32_t last = tot ? tot - 1 : 0u;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // (clamped to the map's entries as well: a tile's range never
-            // leaves them, and a read never leaves the allocation)
-            const uint32_t jc = min(fst + min((uint32_t)(u * 64 + lane), last), ent_last);
-            dp[u] = dps[jc];
-            fr[u] = frs[jc];
+        for (int g = 0; g < G; ++g) {
+            const size_t t = cj * CH + (size_t)g * CSCI_NT + (size_t)wave * 64 + lane;
+            a.n[g] = t < nout ? m_cnt[g] : 0u;
+            const uint32_t x = wave_incl_scan(a.n[g]);
+            a.s[g] = x - a.n[g];
+            a.tot[g] = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+            a.fst[g] = m_fst[g];
+            a.pb[g] = m_pb[g];
+            const uint32_t last = a.tot[g] ? a.tot[g] - 1 : 0u;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // (clamped to the map's entries as well: a tile's range never
+                // leaves them, and a read never leaves the allocation)
+                const uint32_t jc = min(a.fst[g] + min((uint32_t)(u * 64 + lane), last), ent_last);
+                a.dp[g][u] = dps[jc];
+                a.fr[g][u] = frs[jc];
+            }
         }
     };
     size_t ci = (size_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (ci >= c_end) return;
+    St c;
     meta(ci);
-    take(ci, c_n, c_s, c_fst, c_pb, c_tot, c_dp, c_fr);
+    take(ci, c);
     meta(ci + step);
     for (; ci < c_end; ci += step) {
-        // fractions of the current tile into the window (every slot is written;
-        // slots past the tile's entries are never read)
+        // fractions of the current tiles into the windows (every slot is
+        // written; slots past a tile's entries are never read)
 #pragma unroll
-        for (int u = 0; u < U; ++u) wl[u * 64 + lane].f = c_fr[u];
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int u = 0; u < U; ++u) win_all[wave][g][u * 64 + lane].f = c.fr[g][u];
         // values (an entry past the tile reads v at the tile's base)
-        TF wv[U];
+        TF wv[G][U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            wv[u] = vk[min(c_pb + ((uint32_t)(u * 64 + lane) < c_tot ? c_dp[u] : 0u), pix_last)];
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                wv[g][u] = vk[min(c.pb[g] + ((uint32_t)(u * 64 + lane) < c.tot[g] ? c.dp[g][u] : 0u), pix_last)];
         // the next tiles' loads, unconditionally (past the end they are unused):
         // the values' wait below leaves them in flight
-        uint32_t n_n, n_s, n_fst, n_pb, n_tot, n_dp[U];
-        float n_fr[U];
-        take(ci + step, n_n, n_s, n_fst, n_pb, n_tot, n_dp, n_fr);
+        St nx;
+        take(ci + step, nx);
         meta(ci + 2 * step);
 #pragma unroll
-        for (int u = 0; u < U; ++u) wl[u * 64 + lane].v = wv[u];
-        const uint32_t j0 = c_s, j1 = c_s + c_n;
-        TF acc = (TF)0;
-        for (uint32_t j = j0; j < min(j1, (uint32_t)C16_WIN); ++j) {
-            const C16Ent<TF> q = wl[j];
-            acc = fmat((TF)q.f, q.v, acc);
-        }
-        // tiles beyond one window (wave-uniform)
-        for (uint32_t wb = C16_WIN; wb < c_tot; wb += C16_WIN) {
+        for (int g = 0; g < G; ++g) {
+            C16Ent<TF> *wl = win_all[wave][g];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t j = wb + (uint32_t)(u * 64 + lane);
-                if (j < c_tot) wl[u * 64 + lane] = C16Ent<TF>{frs[c_fst + j], vk[c_pb + (uint32_t)dps[c_fst + j]]};
-            }
-            for (uint32_t j = max(j0, wb); j < min(j1, wb + (uint32_t)C16_WIN); ++j) {
-                const C16Ent<TF> q = wl[j - wb];
+            for (int u = 0; u < U; ++u) wl[u * 64 + lane].v = wv[g][u];
+            const uint32_t j0 = c.s[g], j1 = c.s[g] + c.n[g], tot = c.tot[g], fst = c.fst[g], pb = c.pb[g];
+            TF acc = (TF)0;
+            for (uint32_t j = j0; j < min(j1, (uint32_t)C16_WIN); ++j) {
+                const C16Ent<TF> q = wl[j];
                 acc = fmat((TF)q.f, q.v, acc);
             }
-        }
-        const size_t t = ci * CSCI_NT + threadIdx.x;
-        if (t < nout) ok[t] = acc;
-        c_n = n_n; c_s = n_s; c_fst = n_fst; c_pb = n_pb; c_tot = n_tot;
+            // tiles beyond one window (wave-uniform)
+            for (uint32_t wb = C16_WIN; wb < tot; wb += C16_WIN) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) { c_dp[u] = n_dp[u]; c_fr[u] = n_fr[u]; }
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t j = wb + (uint32_t)(u * 64 + lane);
+                    if (j < tot) wl[u * 64 + lane] = C16Ent<TF>{frs[fst + j], vk[pb + (uint32_t)dps[fst + j]]};
+                }
+                for (uint32_t j = max(j0, wb); j < min(j1, wb + (uint32_t)C16_WIN); ++j) {
+                    const C16Ent<TF> q = wl[j - wb];
+                    acc = fmat((TF)q.f, q.v, acc);
+                }
+            }
+            const size_t t = ci * CH + (size_t)g * CSCI_NT + threadIdx.x;
+            if (t < nout) ok[t] = acc;
+        }
+        c = nx;
     }
 }
 template <class TF>
@@ -1288,7 +1306,7 @@ hipError_t csc_image(const ExpDev *ex, int nexp, const TF *v, size_t nout, TF *o
     };
     const unsigned gx = grid(csci_g<TF>());
     if (any_c16)
-        hipLaunchKernelGGL((k_csc_image16<TF>), dim3(grid(1), nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
+        hipLaunchKernelGGL((k_csc_image16<TF>), dim3(grid(C16_G), nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out);
     if (any_wide)
         hipLaunchKernelGGL((k_csc_image<TF>), dim3(gx, nexp), dim3(CSCI_NT), 0, st, ex, v, nout, out, (int)any_c16);
     return hipGetLastError();
