@@ -452,3 +452,49 @@ def test_radix16_row_plan_matches_oracle(small_ds):
     p.iterate(3)
     assert np.array_equal(p.estimate(f64=True), xs["f32"])
     p.close()
+
+
+def test_radix10_column_plan_matches_oracle(small_ds):
+    """FFT 112 x 200: columns of FH = 200 = 2^3 5^2 take radix-10 edge stages
+    (10 2 10, each a prime-factor 2 x 5 butterfly) in the FP32 column passes,
+    and the MIXED tier's FP64 inverse of the summed adjoint spectrum runs the
+    same plan with FP64 twiddles.  The transforms against numpy, every tier
+    against the FP64 oracle within its usual bound, and the plain form equal
+    to the batched form bit for bit in the MIXED and f32 tiers (the plain
+    MIXED form inverts its summed spectrum with the same plan)."""
+    from oracle import oracle as O
+    from updc_amd._lib import FLAG_PER_EXPOSURE, check
+    shape = (112, 200)
+    print(f"case: FFT {shape}, radix-10 column edges", flush=True)
+    for prec, tol in (("f32", 2e-6), ("f64", 1e-13)):
+        c = _ctx(small_ds, prec, fft=shape)
+        c.prepare()
+        rng = np.random.default_rng(7)
+        img = rng.normal(size=(shape[1], shape[0]))
+        ref = np.fft.rfft2(img)
+        spec = c.fft_r2c(img)
+        err = np.max(np.abs(spec - ref)) / np.max(np.abs(ref))
+        back = c.fft_c2r(ref)
+        err2 = np.max(np.abs(back - img * shape[0] * shape[1])) / (shape[0] * shape[1] * np.max(np.abs(img)))
+        print(f"  {prec}: r2c {err:.2e} c2r {err2:.2e}", flush=True)
+        assert err < tol and err2 < tol, (prec, err, err2)
+        c.close()
+    ref = O.RefSolver(small_ds.W, small_ds.H, oracle_exposures(small_ds), "f64", fft_shape=shape)
+    ref.iterate(3)
+    xs = {}
+    for prec, tol in (("f64", 1e-11), ("mixed", 1e-4), ("f32", 1e-4)):
+        c = _ctx(small_ds, prec, fft=shape)
+        c.prepare()
+        c.iterate(3)
+        xs[prec] = c.estimate(f64=True)
+        st = O.compare_stats(ref.x, xs[prec])
+        print(f"  {prec}: norm_linf {st['norm_linf']:.3e}", flush=True)
+        assert st["norm_linf"] < tol, (prec, st)
+        c.close()
+    for prec in ("mixed", "f32"):
+        p = _ctx(small_ds, prec, fft=shape)
+        check(p.lib.updc_ctx_set_flags(p.h, p.flags | FLAG_PER_EXPOSURE))
+        p.prepare()
+        p.iterate(3)
+        assert np.array_equal(p.estimate(f64=True), xs[prec]), prec
+        p.close()

@@ -107,6 +107,21 @@ template <int P, bool INV, class T> __device__ __forceinline__ void bfly_odd(Cx<
 #pragma unroll
     for (int j = 0; j < P; ++j) v[j] = out[j];
 }
+// Radix 10 as a prime-factor (Good-Thomas) 2 x 5 DFT: input n = (5 n1 + 2 n2)
+// mod 10 and output k = (5 k1 + 6 k2) mod 10 make exp(-+2 pi i n k / 10) the
+// product of the 2- and 5-point kernels, so no twiddle sits between the two
+// butterflies (no rounding beyond theirs).
+template <bool INV, class T> __device__ __forceinline__ void bfly10(Cx<T> *v) {
+    Cx<T> a[5] = {v[0], v[2], v[4], v[6], v[8]};
+    Cx<T> b[5] = {v[5], v[7], v[9], v[1], v[3]};
+    bfly_odd<5, INV>(a);
+    bfly_odd<5, INV>(b);
+    v[0] = a[0] + b[0]; v[5] = a[0] - b[0];
+    v[6] = a[1] + b[1]; v[1] = a[1] - b[1];
+    v[2] = a[2] + b[2]; v[7] = a[2] - b[2];
+    v[8] = a[3] + b[3]; v[3] = a[3] - b[3];
+    v[4] = a[4] + b[4]; v[9] = a[4] - b[4];
+}
 template <bool INV, class T> __device__ __forceinline__ void bfly16(Cx<T> *v) {
     // 16 = 4 x 4: X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 W4^(n1 k1) x[4 n1 + n2]
     constexpr double c[10] = {1.0, 0.9238795325112867561282, 0.7071067811865475244008, 0.3826834323650897717285,
@@ -143,6 +158,7 @@ template <int R, bool INV, class T> __device__ __forceinline__ void bfly(Cx<T> *
     else if constexpr (R == 4) bfly4<INV>(v);
     else if constexpr (R == 8) bfly8<INV>(v);
     else if constexpr (R == 16) bfly16<INV>(v);
+    else if constexpr (R == 10) bfly10<INV>(v);
     else bfly_odd<R, INV>(v);
 }
 

@@ -578,8 +578,13 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
         }
     } else {  // COL_ADJ, COL_ACC
         Cx<TA> acc[XMAX];
+        // (COL_ACC with nexp < 0: the summed spectrum from memory, TA at pitch in_rp)
+        const Cx<TA> *ia = reinterpret_cast<const Cx<TA> *>(in) + (size_t)col * in_rp;
 #pragma unroll
-        for (int j = 0; j < XMAX; ++j) acc[j] = cx<TA>((TA)0, (TA)0);
+        for (int j = 0; j < XMAX; ++j) {
+            const int e = tid + j * NT;
+            acc[j] = (MODE == COL_ACC && nexp < 0 && e < hin) ? ia[e] : cx<TA>((TA)0, (TA)0);
+        }
         for (int k = 0; k < nexp; ++k) {
             __syncthreads();
             load(in + (size_t)k * in_bs);
@@ -750,10 +755,22 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
         }
     } else {  // COL_ADJ, COL_ACC
         Cx<TA> acc[NBE][RE];
+        if (MODE == COL_ACC && nexp < 0) {
+            // the summed spectrum from memory (TA, pitch in_rp): only its inverse runs
+            const Cx<TA> *ia = reinterpret_cast<const Cx<TA> *>(in) + (size_t)col * in_rp;
 #pragma unroll
-        for (int r = 0; r < NBE; ++r)
+            for (int r = 0; r < NBE; ++r)
 #pragma unroll
-            for (int q = 0; q < RE; ++q) acc[r][q] = cx<TA>((TA)0, (TA)0);
+                for (int q = 0; q < RE; ++q) {
+                    const int i = tid + r * NT, n = i + q * LR;
+                    acc[r][q] = (i < LR && n < hin) ? ia[n] : cx<TA>((TA)0, (TA)0);
+                }
+        } else {
+#pragma unroll
+            for (int r = 0; r < NBE; ++r)
+#pragma unroll
+                for (int q = 0; q < RE; ++q) acc[r][q] = cx<TA>((TA)0, (TA)0);
+        }
         for (int k = 0; k < nexp; ++k) {
             load(rb, in + (size_t)k * in_bs + (size_t)col * in_rp, hin);
             Cx<T> pf[NBE][RE];
@@ -1039,10 +1056,16 @@ static constexpr size_t LDS_BUDGET = 150 * 1024;
 // and for the interleaved row lines an even R (8: 256-byte strides) sends a
 // wave's writes to one LDS bank window, while an odd R spreads them.
 static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullptr, bool odd_first = false,
-                           bool sixteen = false) {
+                           bool sixteen = false, bool ten = false) {
     int n = L, k = 0;
     int rad[UPDC_MAX_STAGES + 8];
     auto push = [&](int r) { if (k < UPDC_MAX_STAGES) rad[k++] = r; };
+    // ten (with edge): radix-10 edge stages where L has two factors 2 and two
+    // factors 5 (5600 = 10 8 7 10: four stages instead of 5 8 4 7 5)
+    if (edge && ten && L % 100 == 0) {
+        push(10);
+        n /= 100;
+    }
     // sixteen: radix 16 where it saves a stage on the power of two (2^4 = 16
     // against 8 2, 2^7 = 16 8 against 8 8 2); the caller checks it fits
     int p2 = 0;
@@ -1055,7 +1078,11 @@ static bool factor_radices(int L, FftLen &f, bool edge = false, int *re = nullpt
     while (n % 5 == 0) { push(5); n /= 5; }
     while (n % 3 == 0) { push(3); n /= 3; }
     if (re) *re = 0;
-    if (edge && k >= 2) {
+    if (k >= 1 && rad[0] == 10) {
+        // the second radix-10 edge goes last
+        push(10);
+        if (re) *re = 10;
+    } else if (edge && k >= 2) {
         for (int cand : {8, 5, 4, 7, 3}) {
             int cnt = 0;
             for (int i = 0; i < k; ++i) cnt += rad[i] == cand;
@@ -1197,6 +1224,9 @@ template <class T, class TA, int MODE, bool PF, bool PP> static void *col_e_fn_p
         case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF, EV, PP>;
         case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF, EV, PP>;
         case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF, EV, PP>;
+        case 10:
+            if constexpr (EV >= 10) return (void *)k_col_e<T, TA, NT, V, 10, MODE, PF, EV, PP>;
+            return nullptr;
         case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF, EV, PP>;
         default: return nullptr;
     }
@@ -1301,8 +1331,15 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     // one column per workgroup, else four-step columns.  The edge kernel holds
     // EV / RE butterflies of each thread in registers; the LDS-only column
     // kernel keeps XMAX values per thread in registers
-    long_col = !factor_radices(FH, col, allow_edge, &col_re);
+    // radix-10 edges (FP32 plans, where the edge kernel holds them)
+    long_col = !factor_radices(FH, col, allow_edge, &col_re, false, false, allow_ten && sizeof(T) == 4);
+    if (!long_col && col_re == 10 && FH / 10 > (ColCfg<T>::EV / 10) * ColCfg<T>::NT)
+        long_col = !factor_radices(FH, col, allow_edge, &col_re);
     if (!long_col && col_re && FH / col_re > (ColCfg<T>::EV / col_re) * ColCfg<T>::NT) col_re = 0;
+    // radix 10 runs only as an edge stage (the LDS stage loops have no radix-10 case)
+    for (int i = 0; i < col.nst && !long_col; ++i)
+        if (col.rad[i] == 10 && (col_re != 10 || (i != 0 && i != col.nst - 1)))
+            return "internal: radix 10 outside the column plan's edge stages";
     long_col = long_col || (size_t)FH * sizeof(Cx<T>) > LDS_BUDGET || (col_re == 0 && FH > CNT * CXMAX) ||
                !fits(col, 1, CNT, CVMAX);
     if (long_col) col_re = 0;
@@ -1321,6 +1358,13 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
         tw_col = upload_stage_twiddles<T>(col, err);
         if (!tw_col) return err;
         col.tw = tw_col;
+        if constexpr (sizeof(T) < sizeof(double)) {
+            // the same plan with FP64 twiddles: the accumulate pass's FP64 inverse
+            col_wide = col;
+            tw_col_wide = upload_stage_twiddles<double>(col_wide, err);
+            if (!tw_col_wide) return err;
+            col_wide.tw = tw_col_wide;
+        }
     }
     post = upload_roots<T>(FW, s.M + 1, -1, err);
     if (!post) return err;
@@ -1378,6 +1422,9 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
 template <class T> void Fft2<T>::release() {
     if (tw_row) (void)hipFree(tw_row);
     if (tw_col) (void)hipFree(tw_col);
+    if (tw_col_wide) (void)hipFree(tw_col_wide);
+    tw_col_wide = nullptr;
+    col_wide = FftLen{};
     if (post) (void)hipFree(post);
     tw_row = tw_col = post = nullptr;
     long_free(lrow);
@@ -1524,6 +1571,16 @@ hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, i
             break;
     }
     return hipGetLastError();
+}
+
+template <class T>
+template <class TA>
+hipError_t Fft2<T>::acc_inverse(const Cx<TA> *acc, Cx<TA> *out, int out_rows, int hout, const FftLen &fa,
+                                hipStream_t st) {
+    if (!acc_inverse_ok(fa)) return hipErrorInvalidValue;
+    // the accumulate pass with no exposures, its sum read from acc (FH rows)
+    return colpass_impl<TA>(COL_ACC, reinterpret_cast<const Cx<T> *>(acc), 0, s.FH, reinterpret_cast<Cx<T> *>(out), 0,
+                            out_rows, nullptr, 0, s.FH, hout, -1, fa, (size_t)s.FH * sizeof(Cx<TA>), st);
 }
 
 template <class T> bool Fft2<T>::acc_inverse_ok(const FftLen &fa) const {
@@ -1855,5 +1912,7 @@ template hipError_t Fft2<double>::colpass<double>(int, const Cx<double> *, size_
                                                   const Cx<double> *, size_t, int, int, int, hipStream_t);
 template hipError_t Fft2<float>::colacc<double>(const Cx<float> *, size_t, int, Cx<double> *, int, int,
                                                 const Cx<float> *, size_t, int, int, const FftLen *, hipStream_t);
+template hipError_t Fft2<float>::acc_inverse<double>(const Cx<double> *, Cx<double> *, int, int, const FftLen &,
+                                                     hipStream_t);
 
 }  // namespace updc

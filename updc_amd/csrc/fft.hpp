@@ -63,6 +63,11 @@ template <class T> struct Fft2 {
     int col_re = 0;          // edge radix of the column plan (0: LDS-only column kernel)
     bool col_pp = false;     // edge column kernel with two LDS transform buffers (out-of-place stages)
     bool allow_pp = true;    // set before init(): false keeps the edge column kernel in place
+    bool allow_ten = true;   // set before init(): false keeps radix-10 edges out of FP32 column plans
+    // FP32 only: the column plan with FP64 twiddles (the accumulate pass's FP64
+    // inverse of the summed spectrum runs it)
+    FftLen col_wide{};
+    void *tw_col_wide = nullptr;
     bool allow_edge = true;  // set before init(): false forces the LDS-only column kernel
     bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit
     bool row_odd_first = true;   // set before init(): an odd radix first in the row plan (LDS banks)
@@ -133,6 +138,10 @@ template <class T> struct Fft2 {
     // Whether colacc can run the TA inverse with plan fa (same radices as the
     // edge kernel's plan, registers and LDS within the column configuration).
     bool acc_inverse_ok(const FftLen &fa) const;
+    // out = IFFT(acc) in TA with plan fa, the inverse colacc runs, for a sum
+    // already in memory (FH rows, pitch rp(FH)); rows [0, hout), pitch rp(out_rows).
+    template <class TA>
+    hipError_t acc_inverse(const Cx<TA> *acc, Cx<TA> *out, int out_rows, int hout, const FftLen &fa, hipStream_t st);
     template <class TA>
     hipError_t colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
                             size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,

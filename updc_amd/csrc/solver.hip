@@ -382,7 +382,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             if (acc_fused) {
                 // the column inverse of the sum runs in TA inside the accumulating pass
                 HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_xa.p, c.H, c.H, psf_spec.p, fstride, c.H,
-                                               n, &fa.col, c.st));
+                                               n, &fft.col_wide, c.st));
             } else {
                 HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_acc.p, s.FH, s.FH, psf_spec.p, fstride,
                                                c.H, n, nullptr, c.st));
@@ -726,7 +726,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         spec_k.alloc((size_t)n * fft.spec_elems(c.H));
         fused_update = kWide ? fa.row_persistent : fft.row_persistent;
         if (kWide) {
-            acc_fused = fft.acc_inverse_ok(fa.col);
+            acc_fused = fft.acc_inverse_ok(fft.col_wide);
             if (!acc_fused) spec_acc.alloc(fft.spec_elems(s.FH));
             spec_xa.alloc(fa.spec_elems(c.H));
         }
@@ -781,8 +781,13 @@ template <class TF, class TA> struct Pipe : PipeBase {
         }
         mark(2);
         if constexpr (kWide) {
-            HIPCHK(fa.template colpass<TA>(COL_INV, spec_acc.p, 0, s.FH, spec_xa.p, 0, c.H, nullptr, 0, s.FH, c.H, 1,
-                                           c.st));
+            // the column inverse of the sum with the batched form's plan (the
+            // accumulate pass's own), so both forms agree bit for bit
+            if (acc_fused)
+                HIPCHK(fft.template acc_inverse<TA>(spec_acc.p, spec_xa.p, c.H, c.H, fft.col_wide, c.st));
+            else
+                HIPCHK(fa.template colpass<TA>(COL_INV, spec_acc.p, 0, s.FH, spec_xa.p, 0, c.H, nullptr, 0, s.FH,
+                                               c.H, 1, c.st));
             HIPCHK(fa.row_c2r(spec_xa.p, 0, c.H, C.p, 0, c.W, c.W, scale_a(c), c.H, 1, c.st));
         } else {
             HIPCHK(fft.template spec_cast<TA>(spec_acc.p, spec_tmp.p, c.st));
