@@ -56,6 +56,11 @@ extern "C" {
                                        R2C (the S_k^T wr_k images are never stored), instead
                                        of the default: the gather as its own launch for all
                                        exposures, then a plain row R2C of the images */
+#define UPDC_FLAG_COLUMN_SPECTRA 32u /* batched form: the spectra a column pass writes for a row
+                                        C2R (blurred spectra, correction) column-major, instead
+                                        of the default where the passes allow it: column pairs
+                                        interleaved element by element (64-byte row-pass reads);
+                                        results are identical bit for bit */
 
 /* ---- LinkMap ABI (updc/cache_abi.py:9-12, updc_core.c:44-54) -------------- */
 #define UPDC_LINKMAP_MAGIC 0x324D4C55u
@@ -186,6 +191,11 @@ int updc_ctx_set_setup_cache(updc_ctx *ctx, const char *dir);
 /* Caches the last updc_prepare read: bit 0 PSF spectra, bit 1 initial
  * estimate and coverage, bit 2 denominator. */
 int updc_get_setup_cache_hits(updc_ctx *ctx, int *hits);
+/* Layout of the spectra the batched iteration's column passes write, under
+ * the context's current flags (prepared context): 0 column-major, 1 column
+ * pairs interleaved (see UPDC_FLAG_COLUMN_SPECTRA).  No reference counterpart: the reference's
+ * spectra are FFTW's row-major half spectra, internal to each transform. */
+int updc_get_spectrum_layout(updc_ctx *ctx, int *layout);
 
 /* Column kernel of the 2-D FFT: -1 automatic (edge-stage where the column
  * length allows), 0 LDS-only, 1 edge-stage where the length allows.  Set

@@ -454,6 +454,31 @@ def test_radix16_row_plan_matches_oracle(small_ds):
     p.close()
 
 
+def test_column_pair_layout_matches_column_major(small_ds):
+    """The spectra the batched column passes write for the row C2R (blurred
+    spectra, correction) in the column-pair layout (the default where the
+    passes allow it) and column-major (UPDC_FLAG_COLUMN_SPECTRA) feed the
+    same values to the same arithmetic: identical estimates and FreezeMu rows
+    in every tier, under hipGraph replay, for an odd spectrum column count
+    (FFT 112 wide: 57 columns, the last pair with a phantom partner) and an
+    even one (126 wide: 64 columns); FFT height 200 runs the edge column
+    kernel."""
+    print("case: column-pair layout vs column-major, FFT 112 x 200 and 126 x 200", flush=True)
+    for shape in ((112, 200), (126, 200)):
+        for prec in ("f64", "mixed", "f32"):
+            out = []
+            for column in (False, True):
+                c = _ctx(small_ds, prec, fft=shape, graph=True)
+                c.set_column_spectra(column)
+                c.prepare()
+                assert c.spectrum_layout() == ("columns" if column else "pairs"), (shape, prec, column)
+                rows, _ = c.iterate(3)
+                out.append((c.estimate(f64=True), [tuple(r[:4]) for r in rows]))
+                c.close()
+            assert np.array_equal(out[0][0], out[1][0]), (shape, prec)
+            assert out[0][1] == out[1][1], (shape, prec)
+
+
 def test_radix10_column_plan_matches_oracle(small_ds):
     """FFT 112 x 200: columns of FH = 200 = 2^3 5^2 take radix-10 edge stages
     (10 2 10, each a prime-factor 2 x 5 butterfly) in the FP32 column passes,

@@ -21,6 +21,7 @@ FLAG_QUIET = 2
 FLAG_PER_EXPOSURE = 4
 FLAG_AUTOTUNE = 8
 FLAG_FUSED_ADJOINT = 16
+FLAG_COLUMN_SPECTRA = 32
 
 
 class UpdcError(RuntimeError):
@@ -72,6 +73,7 @@ SIGNATURES = {
     "updc_ctx_set_column_kernel": (C.c_int, [_P, C.c_int]),
     "updc_ctx_set_setup_cache": (C.c_int, [_P, C.c_char_p]),
     "updc_get_setup_cache_hits": (C.c_int, [_P, _I]),
+    "updc_get_spectrum_layout": (C.c_int, [_P, _I]),
     "updc_get_autotune": (C.c_int, [_P, _I, _I, _I, _D, C.c_int, _I, _I]),
     "updc_psf_project": (C.c_int, [C.c_int, _F, C.c_int, C.c_int, _D, _F, C.c_size_t, _I, _I, _D, _D,
                                    C.POINTER(C.c_uint64)]),

@@ -426,6 +426,14 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
     const int t = (bid % 8) * per + bid / 8;
     return t < ntiles ? t : -1;
 }
+// As xcd_tile, with every XCD's range starting at an even tile, so that
+// tiles 2p and 2p + 1 run on one XCD, back to back.  Needs
+// 16 * ceil(ntiles / 16) workgroups.
+__device__ __forceinline__ int xcd_tile_even(int bid, int ntiles) {
+    const int per = (ntiles + 15) / 16 * 2;
+    const int t = (bid % 8) * per + bid / 8;
+    return t < ntiles ? t : -1;
+}
 
 // ---------------------------------------------------------------- ordered keys
 // Monotone float/double -> unsigned maps for atomicMin of signed values.

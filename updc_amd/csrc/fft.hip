@@ -56,6 +56,41 @@ template <class T> __device__ __forceinline__ void store_pair(T *row, int c, int
     }
 }
 
+// SL_PAIR loads of the row C2R (SpecLayout in fft.hpp): a lane reads 16
+// bytes, both columns of a pair (FP32) or one of them (FP64, two lanes per
+// pair and row), so a tile's lanes cover whole pair runs.
+template <class T> constexpr int pair_cpl() { return sizeof(T) == 4 ? 2 : 1; }  // complex values per lane
+template <class T> constexpr int pair_lpp() { return 2 / pair_cpl<T>(); }       // lanes per pair and row
+template <class T> __device__ __forceinline__ void load_cpl(const Cx<T> *p, Cx<T> *v) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4 *>(p);
+        v[0] = cx<float>(q.x, q.y);
+        v[1] = cx<float>(q.z, q.w);
+    } else {
+        v[0] = p[0];
+    }
+}
+
+// Real split of RB transformed rows in LDS (row j at slot (n << LG) + j) into
+// the half spectrum X[v] = (Z[v] + Z*[M-v])/2 - i/2 exp(-2 pi i v/FW)
+// (Z[v] - Z*[M-v]), v in [0, M], stored column-major at rows r0 + j < rows.
+template <class T, int NT, int RB>
+__device__ __forceinline__ void split_store(const Cx<T> *z, const Cx<T> *__restrict__ post, Cx<T> *__restrict__ o,
+                                            int M, int RP, int r0, int rows) {
+    constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
+    const T half = (T)0.5;
+    for (int idx = threadIdx.x; idx < ((M + 1) << LG); idx += NT) {
+        const int v = idx >> LG, j = idx & (RB - 1);
+        if (r0 + j >= rows) continue;
+        const Cx<T> zv = z[((v == M ? 0 : v) << LG) + j];
+        const Cx<T> zm = cconj(z[((v == 0 ? 0 : M - v) << LG) + j]);
+        const Cx<T> fe = cscale(zv + zm, half);
+        const Cx<T> d = zv - zm;
+        const Cx<T> fo = cx<T>(d.im * half, -d.re * half);  // (zv - zm) / (2i)
+        o[(size_t)v * RP + r0 + j] = fe + cmul(post[v], fo);
+    }
+}
+
 // Minimum waves per SIMD requested of the row kernels.  Their 1024-thread
 // workgroups (one per CU: 89.6 KB of LDS at M = 2800) already hold them to
 // four waves per SIMD and 128 VGPRs; the request adds no further limit.
@@ -144,19 +179,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c(FftLen
     }
     __syncthreads();
     fft_lds<false, T, NT, VMAX>(z, f, LG);  // the RB rows together
-    // Split: X[v] = (Z[v] + Z*[M-v])/2 - i/2 exp(-2 pi i v/FW) (Z[v] - Z*[M-v]), v in [0, M].
-    Cx<T> *o = out + (size_t)b * out_bstride;
-    const T half = (T)0.5;
-    for (int idx = threadIdx.x; idx < ((M + 1) << LG); idx += NT) {
-        const int v = idx >> LG, j = idx & (RB - 1);
-        if (r0 + j >= rows) continue;
-        const Cx<T> zv = z[((v == M ? 0 : v) << LG) + j];
-        const Cx<T> zm = cconj(z[((v == 0 ? 0 : M - v) << LG) + j]);
-        const Cx<T> fe = cscale(zv + zm, half);
-        const Cx<T> d = zv - zm;
-        const Cx<T> fo = cx<T>(d.im * half, -d.re * half);  // (zv - zm) / (2i)
-        o[(size_t)v * RP + r0 + j] = fe + cmul(post[v], fo);
-    }
+    split_store<T, NT, RB>(z, post, out + (size_t)b * out_bstride, M, RP, r0, rows);
 }
 
 // Row C2R: RB rows per workgroup.  The RB-element column runs of the
@@ -252,31 +275,54 @@ template <> struct RowPL<float> { static constexpr bool value = true; };
 template <> struct RowPL<double> { static constexpr bool value = false; };
 
 // C2R tiles of a persistent workgroup (as k_row_c2r).  PFM complex values per
-// thread cover (M + 1) * RB slots.  emit(b, r0, z) consumes each transformed
+// thread cover (M + 1) * RB slots (SL_COL) or (M + 2) / 2 pairs of RB rows
+// (CP: SL_PAIR spectra).  emit(b, r0, z) consumes each transformed
 // tile from LDS (z[(n << LG) + j] = columns 2n, 2n+1 of row r0 + j, unscaled).
-template <class T, int NT, int VMAX, int RB, int PFM, class Emit>
+template <class T, int NT, int VMAX, int RB, int PFM, bool CP, class Emit>
 __device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *twl, const Cx<T> *__restrict__ post,
                                               const Cx<T> *__restrict__ in, size_t in_bstride, int RP, int rows,
                                               int batch, Cx<T> *z, Emit &emit) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
+    // CP (SL_PAIR spectra): a slot is CPL complex values of one pair and row
+    constexpr int CPL = CP ? pair_cpl<T>() : 1, LPP = CP ? pair_lpp<T>() : 1, NS = PFM / CPL;
     const int M = f.L, nblk = (rows + RB - 1) / RB, nsl = (M + 1) << LG;
     int t, t_end, step;
     row_tiles(nblk * batch, t, t_end, step);
     Cx<T> pre[PFM];
-    // slot idx = tid + m NT holds column v = idx / RB, row j = idx % RB; NT is a
-    // multiple of RB, so j is fixed per thread and v advances by NT / RB per slot
-    const int jt = threadIdx.x & (RB - 1);
-    const size_t vstep = (size_t)(NT >> LG) * RP;
-    // prefetch registers [m0, m1) of tile tt
+    // !CP: slot idx = tid + m NT holds column v = idx / RB, row j = idx % RB; NT
+    // is a multiple of RB, so j is fixed per thread and v advances by NT / RB
+    // per slot.  CP: slot idx holds lane part h = idx % LPP of pair
+    // p = idx / (LPP RB), row j = (idx / LPP) % RB; h and j are fixed per
+    // thread and p advances by NT / (LPP RB) per slot.
+    const int jt = CP ? (threadIdx.x / LPP) & (RB - 1) : threadIdx.x & (RB - 1);
+    const int ht = threadIdx.x % LPP, ut = threadIdx.x / (LPP * RB);
+    constexpr int USTEP = NT / (LPP * RB);
+    const int np = (M + 2) >> 1;  // CP: pairs per row
+    const size_t vstep = (size_t)(NT >> LG) * RP, pstep = (size_t)USTEP * RP * 2;
+    // prefetch slots [m0, m1) of tile tt
     auto load = [&](int tt, int m0, int m1) {
         const int b = tt / nblk, r0 = (tt - b * nblk) * RB;
-        const Cx<T> *p0 = in + (size_t)b * in_bstride + (size_t)(threadIdx.x >> LG) * RP + r0 + jt;
         const bool rok = r0 + jt < rows;
+        if constexpr (CP) {
+            const Cx<T> *p0 = in + (size_t)b * in_bstride + (size_t)ut * RP * 2 + (size_t)(r0 + jt) * 2 + ht * CPL;
 #pragma unroll
-        for (int m = 0; m < PFM; ++m) {
-            if (m < m0 || m >= m1) continue;
-            const int idx = threadIdx.x + m * NT;
-            pre[m] = (idx < nsl && rok) ? p0[m * vstep] : cx<T>((T)0, (T)0);
+            for (int m = 0; m < NS; ++m) {
+                if (m < m0 || m >= m1) continue;
+                if (ut + m * USTEP < np && rok) {
+                    load_cpl(p0 + m * pstep, pre + m * CPL);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) pre[m * CPL + c] = cx<T>((T)0, (T)0);
+                }
+            }
+        } else {
+            const Cx<T> *p0 = in + (size_t)b * in_bstride + (size_t)(threadIdx.x >> LG) * RP + r0 + jt;
+#pragma unroll
+            for (int m = 0; m < PFM; ++m) {
+                if (m < m0 || m >= m1) continue;
+                const int idx = threadIdx.x + m * NT;
+                pre[m] = (idx < nsl && rok) ? p0[m * vstep] : cx<T>((T)0, (T)0);
+            }
         }
     };
     // The next tile's prefetch is issued in thirds: before the merge, after
@@ -285,16 +331,29 @@ __device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *twl,
     // queue and stall it at the issue: 0.65 -> 0.58 ms for 8 x 4096 rows at
     // M = 2800, against 0.62 in halves and 0.63-0.67 in four or six parts.
     // FP64 tiles issue theirs at once: split, the FP64 update kernel spills.
-    constexpr int P3 = sizeof(T) == 4 ? (PFM + 2) / 3 : PFM;
-    if (t < t_end) load(t, 0, PFM);
+    constexpr int P3 = sizeof(T) == 4 ? (NS + 2) / 3 : NS;
+    if (t < t_end) load(t, 0, NS);
     for (; t < t_end; t += step) {
         const int b = t / nblk, r0 = (t - b * nblk) * RB;
         const bool more = t + step < t_end;
         __syncthreads();  // the previous tile's LDS reads are done
+        if constexpr (CP) {
 #pragma unroll
-        for (int m = 0; m < PFM; ++m) {
-            const int idx = threadIdx.x + m * NT;
-            if (idx < nsl) z[idx] = pre[m];
+            for (int m = 0; m < NS; ++m) {
+                const int u = ut + m * USTEP;
+                if (u >= np) continue;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const int v = 2 * u + ht * CPL + c;
+                    if (v <= M) z[(v << LG) + jt] = pre[m * CPL + c];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < PFM; ++m) {
+                const int idx = threadIdx.x + m * NT;
+                if (idx < nsl) z[idx] = pre[m];
+            }
         }
         if (more) load(t + step, 0, P3);
         __syncthreads();
@@ -314,14 +373,14 @@ __device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *twl,
         __syncthreads();
         if (more) load(t + step, P3, 2 * P3);
         auto third = [&](int s) __attribute__((always_inline)) {
-            if (s == 0 && more) load(t + step, 2 * P3, PFM);
+            if (s == 0 && more) load(t + step, 2 * P3, NS);
         };
         fft_lds_hook<true, T, NT, VMAX>(z, f, twl, LG, -1, third);
         emit(b, r0, z);
     }
 }
 
-template <class T, int NT, int VMAX, int RB, int PFM>
+template <class T, int NT, int VMAX, int RB, int PFM, bool CP>
 __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
     FftLen f, const Cx<T> *__restrict__ post, int ntw, const Cx<T> *__restrict__ in, size_t in_bstride, int RP,
     T *__restrict__ out, size_t out_bstride, int out_pitch, int wout, T scale, int rows, int batch, int vec) {
@@ -343,7 +402,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
             }
         }
     };
-    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, twl, pl, in, in_bstride, RP, rows, batch, z, emit);
+    row_c2r_tiles<T, NT, VMAX, RB, PFM, CP>(f, twl, pl, in, in_bstride, RP, rows, batch, z, emit);
 }
 
 // Persistent C2R of the summed adjoint spectrum (batch 1) with the update
@@ -353,7 +412,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_p(
 // signed sum and the negative count: per-thread sums over the thread's
 // elements, then per workgroup, then the last workgroup to finish (ticket)
 // adds the workgroup partials in workgroup order.
-template <class T, class TF, int NT, int VMAX, int RB, int PFM>
+template <class T, class TF, int NT, int VMAX, int RB, int PFM, bool CP>
 __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
     FftLen f, const Cx<T> *__restrict__ post, int ntw, const Cx<T> *__restrict__ in, int RP, TF *__restrict__ x,
     const T *__restrict__ D, int W, T scale, T max_sens, int rows, IterDiag *idiag, double *partials,
@@ -412,7 +471,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
             }
         }
     };
-    row_c2r_tiles<T, NT, VMAX, RB, PFM>(f, twl, pl, in, 0, RP, rows, 1, z, emit);
+    row_c2r_tiles<T, NT, VMAX, RB, PFM, CP>(f, twl, pl, in, 0, RP, rows, 1, z, emit);
 
     if (bad) {
         atomicOr(&idiag->err, bad);
@@ -494,7 +553,6 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c_p(
         }
     };
     if (t < t_end) load(t);
-    const T half = (T)0.5;
     for (; t < t_end; t += step) {
         const int b = t / nblk, r0 = (t - b * nblk) * RB;
         __syncthreads();
@@ -508,18 +566,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c_p(
         if (t + step < t_end) load(t + step);
         __syncthreads();
         fft_lds<false, T, NT, VMAX>(z, f, LG);
-        // Split: X[v] = (Z[v] + Z*[M-v])/2 - i/2 exp(-2 pi i v/FW) (Z[v] - Z*[M-v]), v in [0, M].
-        Cx<T> *o = out + (size_t)b * out_bstride;
-        for (int idx = threadIdx.x; idx < ((M + 1) << LG); idx += NT) {
-            const int v = idx >> LG, j = idx & (RB - 1);
-            if (r0 + j >= rows) continue;
-            const Cx<T> zv = z[((v == M ? 0 : v) << LG) + j];
-            const Cx<T> zm = cconj(z[((v == 0 ? 0 : M - v) << LG) + j]);
-            const Cx<T> fe = cscale(zv + zm, half);
-            const Cx<T> d = zv - zm;
-            const Cx<T> fo = cx<T>(d.im * half, -d.re * half);
-            o[(size_t)v * RP + r0 + j] = fe + cmul(post[v], fo);
-        }
+        split_store<T, NT, RB>(z, post, out + (size_t)b * out_bstride, M, RP, r0, rows);
     }
 }
 
@@ -662,7 +709,7 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
 // global memory: read from LDS, with a butterfly's R-1 twiddles adjacent,
 // they add bank-conflicted LDS traffic to passes that LDS bounds already, and
 // the broadcast and accumulate passes slowed by 10 %.)
-template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF, int EV, bool PP>
+template <class T, class TA, int NT, int VMAX, int RE, int MODE, bool PF, int EV, bool PP, bool CP>
 __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__restrict__ in, size_t in_bs, int in_rp,
                                               Cx<T> *__restrict__ out, size_t out_bs, int out_rp,
                                               const Cx<T> *__restrict__ psf, size_t psf_bs, int psf_rp, int hin,
@@ -670,8 +717,16 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
     constexpr int NBE = EV / RE;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *buf = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int col = xcd_tile(blockIdx.x, nc);
+    // CP (output in SL_PAIR): the two columns of a pair run on one XCD, back
+    // to back, so its L2 merges their halves of every line they write
+    const int col = CP ? xcd_tile_even(blockIdx.x, nc) : xcd_tile(blockIdx.x, nc);
     if (col < 0) return;
+    // column col of an output spectrum with pitch rpitch: its first element;
+    // el(c, n): its element n
+    auto cbase = [&](auto *p, int rpitch) __attribute__((always_inline)) {
+        return CP ? p + (size_t)(col >> 1) * rpitch * 2 + (col & 1) : p + (size_t)col * rpitch;
+    };
+    auto el = [](auto *c, int n) __attribute__((always_inline)) -> auto & { return CP ? c[(size_t)n * 2] : c[n]; };
     const int LR = f.L / RE, ns = f.nst, tid = threadIdx.x;
     const Cx<T> *twl = (const Cx<T> *)f.tw + f.to[ns - 1];
     Cx<T> rb[NBE][RE];
@@ -694,7 +749,7 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
 #pragma unroll
             for (int q = 0; q < RE; ++q) {
                 const int i = tid + r * NT, n = i + q * LR;
-                if (i < LR && n < hout) c[n] = rb[r][q];
+                if (i < LR && n < hout) el(c, n) = rb[r][q];
             }
     };
     auto fwd = [&]() {
@@ -731,7 +786,7 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
         load(rb, in + (size_t)b * in_bs + (size_t)col * in_rp, hin);
         if (MODE == COL_FWD) fwd();
         else inv();
-        store(out + (size_t)b * out_bs + (size_t)col * out_rp);
+        store(cbase(out + (size_t)b * out_bs, out_rp));
     } else if (MODE == COL_BCAST) {
         load(rb, in + (size_t)col * in_rp, hin);
         Cx<T> pf[NBE][RE];
@@ -751,7 +806,7 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
                 for (int q = 0; q < RE; ++q) rb[r][q] = cmul(xr[r][q], pf[r][q]);
             if (PF && k + 1 < nexp) load(pf, psf + (size_t)(k + 1) * psf_bs + (size_t)col * psf_rp, f.L);
             inv();
-            store(out + (size_t)k * out_bs + (size_t)col * out_rp);
+            store(cbase(out + (size_t)k * out_bs, out_rp));
         }
     } else {  // COL_ADJ, COL_ACC
         Cx<TA> acc[NBE][RE];
@@ -790,7 +845,7 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
                 }
         }
         if (MODE == COL_ACC) {
-            Cx<TA> *o = reinterpret_cast<Cx<TA> *>(out) + (size_t)col * out_rp;
+            Cx<TA> *o = cbase(reinterpret_cast<Cx<TA> *>(out), out_rp);
             if (fa.L > 0) {
                 // the inverse of the sum in TA (plan fa: the same radices, TA
                 // twiddles) over the whole LDS allocation, FH TA values; the
@@ -809,7 +864,7 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
 #pragma unroll
                 for (int q = 0; q < RE; ++q) {
                     const int i = tid + r * NT, n = i + q * LR;
-                    if (i < LR && n < hout) o[n] = acc[r][q];
+                    if (i < LR && n < hout) el(o, n) = acc[r][q];
                 }
             return;
         }
@@ -818,7 +873,7 @@ __global__ __launch_bounds__(NT) void k_col_e(FftLen f, int nc, const Cx<T> *__r
 #pragma unroll
             for (int q = 0; q < RE; ++q) rb[r][q] = cx<T>((T)acc[r][q].re, (T)acc[r][q].im);
         inv();
-        store(out + (size_t)col * out_rp);
+        store(cbase(out, out_rp));
     }
 }
 
@@ -1187,11 +1242,11 @@ template <class T, int MODE> static void *row_r2c_fn() {
 template <class T> static int pairs_aligned(const void *p, int pitch, size_t bstride) {
     return p && ((uintptr_t)p % (2 * sizeof(T))) == 0 && pitch % 2 == 0 && bstride % 2 == 0;
 }
-template <class T> static void *row_c2r_p_fn() {
-    return (void *)k_row_c2r_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM>;
+template <class T, bool CP> static void *row_c2r_p_fn() {
+    return (void *)k_row_c2r_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM, CP>;
 }
-template <class T, class TF> static void *row_c2r_upd_fn() {
-    return (void *)k_row_c2r_upd_p<T, TF, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM>;
+template <class T, class TF, bool CP> static void *row_c2r_upd_fn() {
+    return (void *)k_row_c2r_upd_p<T, TF, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM, CP>;
 }
 template <class T> static void *row_r2c_p_fn() {
     return (void *)k_row_r2c_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFR>;
@@ -1217,31 +1272,38 @@ template <class T, class TA, int MODE> static void *col_fn() {
     return (void *)k_col<T, TA, ColCfg<T>::NT, ColCfg<T>::VMAX, ColCfg<T>::XMAX, MODE>;
 }
 
-template <class T, class TA, int MODE, bool PF, bool PP> static void *col_e_fn_pf(int re) {
+template <class T, class TA, int MODE, bool PF, bool PP, bool CP> static void *col_e_fn_pf(int re) {
     constexpr int NT = ColCfg<T>::NT, V = ColCfg<T>::VMAX, EV = ColCfg<T>::EV;
     switch (re) {
-        case 3: return (void *)k_col_e<T, TA, NT, V, 3, MODE, PF, EV, PP>;
-        case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF, EV, PP>;
-        case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF, EV, PP>;
-        case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF, EV, PP>;
+        case 3: return (void *)k_col_e<T, TA, NT, V, 3, MODE, PF, EV, PP, CP>;
+        case 4: return (void *)k_col_e<T, TA, NT, V, 4, MODE, PF, EV, PP, CP>;
+        case 5: return (void *)k_col_e<T, TA, NT, V, 5, MODE, PF, EV, PP, CP>;
+        case 7: return (void *)k_col_e<T, TA, NT, V, 7, MODE, PF, EV, PP, CP>;
         case 10:
-            if constexpr (EV >= 10) return (void *)k_col_e<T, TA, NT, V, 10, MODE, PF, EV, PP>;
+            if constexpr (EV >= 10) return (void *)k_col_e<T, TA, NT, V, 10, MODE, PF, EV, PP, CP>;
             return nullptr;
-        case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF, EV, PP>;
+        case 8: return (void *)k_col_e<T, TA, NT, V, 8, MODE, PF, EV, PP, CP>;
         default: return nullptr;
     }
 }
-// pp: the two-buffer kernel (FP32 only; FP64 columns fill the LDS with one)
-template <class T, class TA, int MODE> static void *col_e_fn(int re, bool pp) {
+// pp: the two-buffer kernel (FP32 only; FP64 columns fill the LDS with one).
+// cp: SL_PAIR in and out (the batched modes only; nullptr for COL_FWD/COL_INV)
+template <class T, class TA, int MODE> static void *col_e_fn(int re, bool pp, bool cp = false) {
     constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ || MODE == COL_ACC);
+    if (cp) {
+        if constexpr (MODE == COL_FWD || MODE == COL_INV) return nullptr;
+        if constexpr (sizeof(T) == 4)
+            if (pp) return col_e_fn_pf<T, TA, MODE, PF, true, true>(re);
+        return col_e_fn_pf<T, TA, MODE, PF, false, true>(re);
+    }
     if constexpr (sizeof(T) == 4)
-        if (pp) return col_e_fn_pf<T, TA, MODE, PF, true>(re);
-    return col_e_fn_pf<T, TA, MODE, PF, false>(re);
+        if (pp) return col_e_fn_pf<T, TA, MODE, PF, true, false>(re);
+    return col_e_fn_pf<T, TA, MODE, PF, false, false>(re);
 }
 
 // COL_ACC kernels exist only for a sum wider than the transform type.
-template <class T, class TA> static void *col_acc_e_fn(int re, bool pp) {
-    if constexpr (sizeof(TA) > sizeof(T)) return col_e_fn<T, TA, COL_ACC>(re, pp);
+template <class T, class TA> static void *col_acc_e_fn(int re, bool pp, bool cp = false) {
+    if constexpr (sizeof(TA) > sizeof(T)) return col_e_fn<T, TA, COL_ACC>(re, pp, cp);
     return nullptr;
 }
 // LDS of the two-buffer column kernel: two lines
@@ -1374,7 +1436,8 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     col_pp = allow_pp && col_re && sizeof(T) == 4 && col_pp_lds<T>(FH, col) <= LDS_BUDGET;
     const int col_lds = (int)(col_pp ? col_pp_lds<T>(FH, col) : (size_t)FH * sizeof(Cx<T>));
     void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_r2c_p_fn<T>()};
-    void *fns_row_p[] = {row_c2r_p_fn<T>(), row_c2r_upd_fn<T, float>(), row_c2r_upd_fn<T, T>()};
+    void *fns_row_p[] = {row_c2r_p_fn<T, false>(), row_c2r_upd_fn<T, float, false>(), row_c2r_upd_fn<T, T, false>(),
+                         row_c2r_p_fn<T, true>(), row_c2r_upd_fn<T, float, true>(), row_c2r_upd_fn<T, T, true>()};
     // persistent row passes: one workgroup per CU, prefetch registers must cover a tile
     {
         int dev = 0;
@@ -1403,7 +1466,9 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     if (col_re) {
         void *fns_e[] = {col_e_fn<T, T, COL_FWD>(col_re, col_pp), col_e_fn<T, T, COL_INV>(col_re, col_pp),
                          col_e_fn<T, T, COL_BCAST>(col_re, col_pp), col_e_fn<T, T, COL_ADJ>(col_re, col_pp),
-                         col_e_fn<T, double, COL_ADJ>(col_re, col_pp)};
+                         col_e_fn<T, double, COL_ADJ>(col_re, col_pp),
+                         col_e_fn<T, T, COL_BCAST>(col_re, col_pp, true), col_e_fn<T, T, COL_ADJ>(col_re, col_pp, true),
+                         col_e_fn<T, double, COL_ADJ>(col_re, col_pp, true)};
         for (void *fn : fns_e)
             if (!fn || hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, col_lds) != hipSuccess)
                 return "cannot raise LDS limit for the column pass";
@@ -1414,6 +1479,10 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
         const int acc_lds = (int)std::min<size_t>((size_t)FH * sizeof(Cx<double>), LDS_BUDGET);
         if (!fn_acc || hipFuncSetAttribute(fn_acc, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            std::max(col_lds, acc_lds)) != hipSuccess)
+            return "cannot raise LDS limit for the column pass";
+        void *fn_acc_cp = col_re ? col_acc_e_fn<T, double>(col_re, col_pp, true) : nullptr;
+        if (fn_acc_cp && hipFuncSetAttribute(fn_acc_cp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             std::max(col_lds, acc_lds)) != hipSuccess)
             return "cannot raise LDS limit for the column pass";
     }
     return "";
@@ -1466,16 +1535,22 @@ hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pit
 
 template <class T>
 hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *out, size_t out_bstride,
-                            int out_pitch, int wout, T scale, int rows, int batch, hipStream_t st) {
+                            int out_pitch, int wout, T scale, int rows, int batch, hipStream_t st, int lin) {
+    if (lin != SL_COL && (lin != SL_PAIR || !row_pair_ok())) return hipErrorNotSupported;
     if (rows <= 0 || batch <= 0) return hipSuccess;
     if (long_row) return row_c2r_long(in, in_bstride, rp_rows, out, out_bstride, out_pitch, wout, scale, rows, batch, st);
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
     const size_t lds = row_lds<T>(s.M);
     const int vec = pairs_aligned<T>(out, out_pitch, out_bstride);
     if (row_persistent) {
-        hipLaunchKernelGGL((k_row_c2r_p<T, NT, VMAX, RB, RowCfg<T>::PFM>), dim3(row_grid), dim3(NT),
-                           row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, in_bstride, rp(rp_rows), out,
-                           out_bstride, out_pitch, wout, scale, rows, batch, vec);
+        if (lin == SL_PAIR)
+            hipLaunchKernelGGL((k_row_c2r_p<T, NT, VMAX, RB, RowCfg<T>::PFM, true>), dim3(row_grid), dim3(NT),
+                               row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, in_bstride, rp(rp_rows), out,
+                               out_bstride, out_pitch, wout, scale, rows, batch, vec);
+        else
+            hipLaunchKernelGGL((k_row_c2r_p<T, NT, VMAX, RB, RowCfg<T>::PFM, false>), dim3(row_grid), dim3(NT),
+                               row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, in_bstride, rp(rp_rows), out,
+                               out_bstride, out_pitch, wout, scale, rows, batch, vec);
         return hipGetLastError();
     }
     const dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
@@ -1488,12 +1563,18 @@ template <class T>
 template <class TF>
 hipError_t Fft2<T>::row_c2r_update(const Cx<T> *in, int rp_rows, TF *x, const T *D, int W, int H, T scale,
                                    T max_sens, IterDiag *idiag, double *partials, unsigned long long *negs,
-                                   unsigned int *ticket, hipStream_t st) {
+                                   unsigned int *ticket, hipStream_t st, int lin) {
     if (!row_persistent) return hipErrorNotSupported;
+    if (lin != SL_COL && (lin != SL_PAIR || !row_pair_ok())) return hipErrorNotSupported;
     constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT, VMAX = RowCfg<T>::VMAX;
-    hipLaunchKernelGGL((k_row_c2r_upd_p<T, TF, NT, VMAX, RB, RowCfg<T>::PFM>), dim3(row_grid), dim3(NT),
-                       row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, rp(rp_rows), x, D, W, scale, max_sens,
-                       H, idiag, partials, negs, ticket);
+    if (lin == SL_PAIR)
+        hipLaunchKernelGGL((k_row_c2r_upd_p<T, TF, NT, VMAX, RB, RowCfg<T>::PFM, true>), dim3(row_grid), dim3(NT),
+                           row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, rp(rp_rows), x, D, W, scale,
+                           max_sens, H, idiag, partials, negs, ticket);
+    else
+        hipLaunchKernelGGL((k_row_c2r_upd_p<T, TF, NT, VMAX, RB, RowCfg<T>::PFM, false>), dim3(row_grid), dim3(NT),
+                           row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, rp(rp_rows), x, D, W, scale,
+                           max_sens, H, idiag, partials, negs, ticket);
     return hipGetLastError();
 }
 
@@ -1501,17 +1582,20 @@ template <class T>
 template <class TA>
 hipError_t Fft2<T>::colpass(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
                             size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
-                            int hout, int batch, hipStream_t st) {
+                            int hout, int batch, hipStream_t st, int lout) {
     FftLen none{};  // no TA inverse
     return colpass_impl<TA>(mode, in, in_bstride, in_rows, out, out_bstride, out_rows, psf, psf_bstride, hin, hout,
-                            batch, none, (size_t)s.FH * sizeof(Cx<T>), st);
+                            batch, none, (size_t)s.FH * sizeof(Cx<T>), st, lout);
 }
 
 template <class T>
 template <class TA>
 hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
                                  size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
-                                 int hout, int batch, FftLen fa, size_t lds, hipStream_t st) {
+                                 int hout, int batch, FftLen fa, size_t lds, hipStream_t st, int lout) {
+    if (lout != SL_COL && lout != SL_PAIR) return hipErrorInvalidValue;
+    const bool cp = lout == SL_PAIR;
+    if (cp && (!col_pair_ok() || mode == COL_FWD || mode == COL_INV)) return hipErrorNotSupported;
     if (long_col) {
         if (fa.L > 0) return hipErrorInvalidValue;  // no fused TA inverse on four-step columns
         return colpass_long<TA>(mode, in, in_bstride, in_rows, out, out_bstride, out_rows, psf, psf_bstride, hin,
@@ -1525,9 +1609,9 @@ hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, i
     if (col_re) {
         void *fn = mode == COL_FWD     ? col_e_fn<T, T, COL_FWD>(col_re, col_pp)
                    : mode == COL_INV   ? col_e_fn<T, T, COL_INV>(col_re, col_pp)
-                   : mode == COL_BCAST ? col_e_fn<T, T, COL_BCAST>(col_re, col_pp)
-                   : mode == COL_ADJ   ? col_e_fn<T, TA, COL_ADJ>(col_re, col_pp)
-                                       : col_acc_e_fn<T, TA>(col_re, col_pp);
+                   : mode == COL_BCAST ? col_e_fn<T, T, COL_BCAST>(col_re, col_pp, cp)
+                   : mode == COL_ADJ   ? col_e_fn<T, TA, COL_ADJ>(col_re, col_pp, cp)
+                                       : col_acc_e_fn<T, TA>(col_re, col_pp, cp);
         if (!fn) return hipErrorInvalidValue;
         const int nc = s.NC, nexp = batch;
         const Cx<T> *pin = in, *ppsf = psf;
@@ -1535,7 +1619,9 @@ hipError_t Fft2<T>::colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, i
         void *args[] = {(void *)&col, (void *)&nc, (void *)&pin, (void *)&in_bstride, (void *)&irp,
                         (void *)&pout, (void *)&out_bstride, (void *)&orp, (void *)&ppsf, (void *)&psf_bstride,
                         (void *)&prp, (void *)&hin, (void *)&hout, (void *)&nexp, (void *)&fa};
-        const dim3 grid(gx, (mode == COL_FWD || mode == COL_INV) ? batch : 1);
+        // SL_PAIR: every XCD's column range starts at an even column (xcd_tile_even)
+        const int gxe = cp ? (s.NC + 15) / 16 * 16 : gx;
+        const dim3 grid(gxe, (mode == COL_FWD || mode == COL_INV) ? batch : 1);
         if (col_pp) lds = std::max(lds, col_pp_lds<T>(s.FH, col));
         return hipLaunchKernel(fn, grid, dim3(CNT), args, lds, st);
     }
@@ -1583,6 +1669,14 @@ hipError_t Fft2<T>::acc_inverse(const Cx<TA> *acc, Cx<TA> *out, int out_rows, in
                             out_rows, nullptr, 0, s.FH, hout, -1, fa, (size_t)s.FH * sizeof(Cx<TA>), st);
 }
 
+template <class T> bool Fft2<T>::row_pair_ok() const {
+    constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT;
+    // the C2R prefetch slots must cover the tile's pairs: (M + 2) / 2 pairs of
+    // RB rows, pair_lpp lanes each, pair_cpl complex values per lane
+    const long slots = (long)(RowCfg<T>::PFM / pair_cpl<T>()) * NT;
+    return row_persistent && !long_row && (long)((s.M + 2) / 2) * RB * pair_lpp<T>() <= slots;
+}
+
 template <class T> bool Fft2<T>::acc_inverse_ok(const FftLen &fa) const {
     constexpr int CNT = ColCfg<T>::NT, CVMAX = ColCfg<T>::VMAX;
     if (long_col || !fa.tw || fa.L != s.FH || (size_t)s.FH * sizeof(Cx<double>) > LDS_BUDGET || !fits(fa, 1, CNT, CVMAX)) return false;
@@ -1598,14 +1692,14 @@ template <class T>
 template <class TA>
 hipError_t Fft2<T>::colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, int out_rows, int hout,
                            const Cx<T> *psf, size_t psf_bstride, int hin, int batch, const FftLen *fa,
-                           hipStream_t st) {
+                           hipStream_t st, int lout) {
     // the kernels write Cx<TA> elements at pitch rp(out_rows) through `out`
     if (fa && !acc_inverse_ok(*fa)) return hipErrorInvalidValue;
     FftLen none{};
     const size_t lds = (size_t)s.FH * (fa ? sizeof(Cx<TA>) : sizeof(Cx<T>));
     return colpass_impl<TA>(COL_ACC, in, in_bstride, in_rows, reinterpret_cast<Cx<T> *>(out), 0,
                             fa ? out_rows : s.FH, psf, psf_bstride, hin, fa ? hout : s.FH, batch, fa ? *fa : none,
-                            lds, st);
+                            lds, st, lout);
 }
 
 template <class T> hipError_t Fft2<T>::spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st) {
@@ -1891,13 +1985,13 @@ template struct Fft2<float>;
 template struct Fft2<double>;
 template hipError_t Fft2<float>::row_c2r_update<float>(const Cx<float> *, int, float *, const float *, int, int,
                                                        float, float, IterDiag *, double *, unsigned long long *,
-                                                       unsigned int *, hipStream_t);
+                                                       unsigned int *, hipStream_t, int);
 template hipError_t Fft2<double>::row_c2r_update<float>(const Cx<double> *, int, float *, const double *, int, int,
                                                         double, double, IterDiag *, double *, unsigned long long *,
-                                                        unsigned int *, hipStream_t);
+                                                        unsigned int *, hipStream_t, int);
 template hipError_t Fft2<double>::row_c2r_update<double>(const Cx<double> *, int, double *, const double *, int,
                                                          int, double, double, IterDiag *, double *,
-                                                         unsigned long long *, unsigned int *, hipStream_t);
+                                                         unsigned long long *, unsigned int *, hipStream_t, int);
 template hipError_t Fft2<float>::spec_acc<float>(const Cx<float> *, const Cx<float> *, Cx<float> *, hipStream_t);
 template hipError_t Fft2<float>::spec_acc<double>(const Cx<float> *, const Cx<float> *, Cx<double> *, hipStream_t);
 template hipError_t Fft2<double>::spec_acc<double>(const Cx<double> *, const Cx<double> *, Cx<double> *, hipStream_t);
@@ -1905,13 +1999,13 @@ template hipError_t Fft2<float>::spec_cast<float>(const Cx<float> *, Cx<float> *
 template hipError_t Fft2<float>::spec_cast<double>(const Cx<double> *, Cx<float> *, hipStream_t);
 template hipError_t Fft2<double>::spec_cast<double>(const Cx<double> *, Cx<double> *, hipStream_t);
 template hipError_t Fft2<float>::colpass<float>(int, const Cx<float> *, size_t, int, Cx<float> *, size_t, int,
-                                                const Cx<float> *, size_t, int, int, int, hipStream_t);
+                                                const Cx<float> *, size_t, int, int, int, hipStream_t, int);
 template hipError_t Fft2<float>::colpass<double>(int, const Cx<float> *, size_t, int, Cx<float> *, size_t, int,
-                                                 const Cx<float> *, size_t, int, int, int, hipStream_t);
+                                                 const Cx<float> *, size_t, int, int, int, hipStream_t, int);
 template hipError_t Fft2<double>::colpass<double>(int, const Cx<double> *, size_t, int, Cx<double> *, size_t, int,
-                                                  const Cx<double> *, size_t, int, int, int, hipStream_t);
+                                                  const Cx<double> *, size_t, int, int, int, hipStream_t, int);
 template hipError_t Fft2<float>::colacc<double>(const Cx<float> *, size_t, int, Cx<double> *, int, int,
-                                                const Cx<float> *, size_t, int, int, const FftLen *, hipStream_t);
+                                                const Cx<float> *, size_t, int, int, const FftLen *, hipStream_t, int);
 template hipError_t Fft2<float>::acc_inverse<double>(const Cx<double> *, Cx<double> *, int, int, const FftLen &,
                                                      hipStream_t);
 

@@ -20,6 +20,24 @@ namespace updc {
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_BCAST = 2, COL_ADJ = 3, COL_ACC = 4 };
 enum RowMode { ROW_PLAIN = 0, ROW_GATHER = 1 };
 
+// Spectrum layouts.  SL_COL: column-major, (r, v) at v * RP + r.  SL_PAIR:
+// the columns in pairs (2p, 2p+1) interleaved element by element, (r, v) at
+// ((v >> 1) * RP + r) * 2 + (v & 1).  A row C2R tile of RB rows then reads
+// RB * 2 consecutive elements per pair (64 bytes for FP32 RB = 4 and FP64
+// RB = 2) instead of RB per column (32 bytes), while the column pass that
+// writes the spectrum stores every other element of a 2 RP run; it runs the
+// two columns of a pair on one XCD, back to back, so its L2 merges their
+// halves of every line.  Measured for 8 spectra of 2801 columns x 4096 rows
+// (FP32, 734 MB): a 4-row tile pass reads in 0.22 ms instead of 0.30, the
+// column pass writes in 0.15 ms instead of 0.13.  The batched iteration
+// uses SL_PAIR for the spectra a column pass writes and a row C2R reads (the
+// blurred spectra, the correction) where both kernels support it
+// (pair_ok).  The spectra a row R2C writes stay SL_COL: the same layout made
+// the adjoint images' row R2C and the summed-adjoint column pass slower
+// together, by 0.06 ms at that size.  PSF spectra, the per-exposure form and
+// the operators are SL_COL throughout.
+enum SpecLayout { SL_COL = 0, SL_PAIR = 1 };
+
 struct FftShape {
     int FW = 0, FH = 0, M = 0, NC = 0;
 };
@@ -95,7 +113,16 @@ template <class T> struct Fft2 {
         const size_t r = long_row ? (size_t)rows * s.M : 0, c = long_col ? (size_t)s.NC * s.FH : 0;
         return std::max(r, c) <= lt_a_n && r <= lt_b_n;
     }
-    size_t spec_elems(int rows) const { return (size_t)s.NC * rp(rows); }
+    // elements of one spectrum buffer in either layout (SL_PAIR stores a
+    // phantom partner of the last column when NC is odd)
+    size_t spec_elems(int rows) const { return (size_t)(s.NC + (s.NC & 1)) * rp(rows); }
+    // Whether the batched passes accept SL_PAIR spectra: the row C2R passes
+    // (row_c2r, row_c2r_update) where they are persistent and no row is
+    // four-step, the column passes' outputs (colpass, colacc) where the edge
+    // column kernel runs.
+    bool row_pair_ok() const;
+    bool col_pair_ok() const { return col_re > 0 && !long_col; }
+    bool pair_ok() const { return row_pair_ok() && col_pair_ok(); }
 
     // Row R2C of rows [0, rows) of `batch` images -> column-major spectra
     // out[b * out_bstride + v * RP + r], RP = rp(rp_rows).
@@ -105,9 +132,10 @@ template <class T> struct Fft2 {
                        const T *wr, int W, Cx<T> *out, size_t out_bstride, int rp_rows, int rows, int batch,
                        hipStream_t st);
     // Row C2R of rows [0, rows) of `batch` column-major spectra (pitch
-    // rp(rp_rows)) -> out[b*out_bstride + r*out_pitch + c] for c < wout, times scale.
+    // rp(rp_rows); lin: their layout) -> out[b*out_bstride + r*out_pitch + c]
+    // for c < wout, times scale.
     hipError_t row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *out, size_t out_bstride, int out_pitch,
-                       int wout, T scale, int rows, int batch, hipStream_t st);
+                       int wout, T scale, int rows, int batch, hipStream_t st, int lin = SL_COL);
     // Row C2R of one spectrum (rows [0, H), pitch rp(rp_rows)) fused with the
     // multiplicative update of x (W x H) by C = scale * C2R against D, the
     // signed sum and negative count going to idiag (partials/negs: one slot
@@ -116,17 +144,19 @@ template <class T> struct Fft2 {
     template <class TF>
     hipError_t row_c2r_update(const Cx<T> *in, int rp_rows, TF *x, const T *D, int W, int H, T scale, T max_sens,
                               IterDiag *idiag, double *partials, unsigned long long *negs, unsigned int *ticket,
-                              hipStream_t st);
+                              hipStream_t st, int lin = SL_COL);
     // Column passes (complex, length FH) over column-major spectra.  Rows >=
     // hin read as zero; rows [0, hout) are stored.  Input pitch rp(in_rows),
     // output pitch rp(out_rows); PSF spectra have pitch rp(FH).
     //   COL_FWD / COL_INV : in[b] -> out[b], b < batch
     //   COL_BCAST: X = FFT(in); out[k] = IFFT(X * psf[k]) for k < nexp
     //   COL_ADJ  : out = IFFT(sum_k conj(psf[k]) * FFT(in[k]))   (sum in TA)
+    // in and the PSF spectra are SL_COL; lout: the layout of out (SL_PAIR
+    // needs col_pair_ok() and a batched mode: COL_BCAST, COL_ADJ, COL_ACC).
     template <class TA>
     hipError_t colpass(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out, size_t out_bstride,
                        int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin, int hout, int batch,
-                       hipStream_t st);
+                       hipStream_t st, int lout = SL_COL);
     // COL_ACC: A = sum_k conj(psf[k]) * FFT(in[k]) kept in TA (TA wider than
     // T).  fa == nullptr: out = A, all FH rows, pitch rp(FH).  Else out =
     // IFFT(A) computed in TA inside the same pass with plan fa (the TA plan of
@@ -134,7 +164,8 @@ template <class T> struct Fft2 {
     // rp(out_rows).
     template <class TA>
     hipError_t colacc(const Cx<T> *in, size_t in_bstride, int in_rows, Cx<TA> *out, int out_rows, int hout,
-                      const Cx<T> *psf, size_t psf_bstride, int hin, int batch, const FftLen *fa, hipStream_t st);
+                      const Cx<T> *psf, size_t psf_bstride, int hin, int batch, const FftLen *fa, hipStream_t st,
+                      int lout = SL_COL);
     // Whether colacc can run the TA inverse with plan fa (same radices as the
     // edge kernel's plan, registers and LDS within the column configuration).
     bool acc_inverse_ok(const FftLen &fa) const;
@@ -145,7 +176,7 @@ template <class T> struct Fft2 {
     template <class TA>
     hipError_t colpass_impl(int mode, const Cx<T> *in, size_t in_bstride, int in_rows, Cx<T> *out,
                             size_t out_bstride, int out_rows, const Cx<T> *psf, size_t psf_bstride, int hin,
-                            int hout, int batch, FftLen fa, size_t lds, hipStream_t st);
+                            int hout, int batch, FftLen fa, size_t lds, hipStream_t st, int lout = SL_COL);
     // Plain (per-exposure) form helpers over full column-major spectra (pitch rp(FH)):
     //   out = a * psf_k ;  acc += conj(psf_k) * a  (acc in TA) ;  out = (T) acc
     hipError_t spec_mul(const Cx<T> *a, const Cx<T> *psf_k, Cx<T> *out, hipStream_t st);

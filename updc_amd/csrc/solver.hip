@@ -249,6 +249,7 @@ struct PipeBase {
     virtual void get_frozen(Ctx &c, int k, uint8_t *out) = 0;
     virtual void reset_frozen(Ctx &c) = 0;
     virtual void invalidate_graph() = 0;  // after a flag change that alters the launches
+    virtual int spectrum_layout(const Ctx &c) const = 0;  // SpecLayout of the batched iteration's spectra
     // The plain form's extra spectra, allocated here (at prepare or when the
     // flag is switched on), never inside launch_iteration_plain, which may
     // run under stream capture where hipMalloc is not allowed.
@@ -349,6 +350,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     DevBuf<Cx<TA>> spec_xa;   // kWide: the sum after its TA column inverse, rows [0, H)
     bool acc_fused = false;   // kWide: that inverse runs inside the accumulating column pass
     bool fused_update = false;  // the final row C2R applies the update (persistent row pass available)
+    int lay = SL_COL;           // layout of the spectra the batched iteration's column passes write
     bool any_c16 = false, any_wide = false;  // exposures with / without the compact CSC
     DevBuf<ExpDiag> diag_cur, diag_hist;
     DevBuf<IterDiag> idiag_cur, idiag_hist;
@@ -375,17 +377,20 @@ template <class TF, class TA> struct Pipe : PipeBase {
     // after each launch but the last.  With update_x the final row C2R also
     // applies the update to x (updc_core.c:1437-1457) where the persistent
     // row pass allows it, and returns true; C is then not stored.
-    template <class Mark> bool adjoint_sum(Ctx &c, int n, Mark mark, bool update_x = false) {
+    // l: the layout the column pass writes its output in (spec_k stays SL_COL).
+    template <class Mark> bool adjoint_sum(Ctx &c, int n, Mark mark, bool update_x, int l) {
         const FftShape &s = fft.s;
         const size_t kstride = fft.spec_elems(c.H), fstride = fft.spec_elems(s.FH);
         if constexpr (kWide) {
+            // the sum leaves the accumulating pass in layout la
+            const int la = acc_fused ? l : SL_COL;
             if (acc_fused) {
                 // the column inverse of the sum runs in TA inside the accumulating pass
                 HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_xa.p, c.H, c.H, psf_spec.p, fstride, c.H,
-                                               n, &fft.col_wide, c.st));
+                                               n, &fft.col_wide, c.st, la));
             } else {
                 HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_acc.p, s.FH, s.FH, psf_spec.p, fstride,
-                                               c.H, n, nullptr, c.st));
+                                               c.H, n, nullptr, c.st, SL_COL));
                 mark();
                 HIPCHK(fa.template colpass<TA>(COL_INV, spec_acc.p, 0, s.FH, spec_xa.p, 0, c.H, nullptr, 0, s.FH, c.H,
                                                1, c.st));
@@ -393,24 +398,28 @@ template <class TF, class TA> struct Pipe : PipeBase {
             mark();
             if (update_x && fused_update) {
                 HIPCHK(fa.template row_c2r_update<TF>(spec_xa.p, c.H, x.p, D.p, c.W, c.H, scale_a(c), (TA)max_sens,
-                                                      idiag_cur.p, partials.p, negs.p, ticket.p, c.st));
+                                                      idiag_cur.p, partials.p, negs.p, ticket.p, c.st, la));
                 return true;
             }
-            HIPCHK(fa.row_c2r(spec_xa.p, 0, c.H, C.p, 0, c.W, c.W, scale_a(c), c.H, 1, c.st));
+            HIPCHK(fa.row_c2r(spec_xa.p, 0, c.H, C.p, 0, c.W, c.W, scale_a(c), c.H, 1, c.st, la));
         } else {
             HIPCHK(fft.template colpass<TA>(COL_ADJ, spec_k.p, kstride, c.H, spec_x.p, 0, c.H, psf_spec.p, fstride,
-                                            c.H, c.H, n, c.st));
+                                            c.H, c.H, n, c.st, l));
             mark();
             if (update_x && fused_update) {
                 HIPCHK(fft.template row_c2r_update<TF>(spec_x.p, c.H, x.p, (const TF *)D.p, c.W, c.H, scale(c),
                                                        (TF)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p,
-                                                       c.st));
+                                                       c.st, l));
                 return true;
             }
-            HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st));
+            HIPCHK(fft.row_c2r(spec_x.p, 0, c.H, C.p, 0, c.W, c.W, scale(c), c.H, 1, c.st, l));
         }
         return false;
     }
+    // The layout of the spectra the batched iteration's column passes write,
+    // under the context's flags: SL_PAIR where prepare found the passes able
+    // to use it, unless UPDC_FLAG_COLUMN_SPECTRA asks for SL_COL.
+    int iter_layout(const Ctx &c) const { return (c.flags & UPDC_FLAG_COLUMN_SPECTRA) ? SL_COL : lay; }
 
     // ---- PSF spectra P^_k for every exposure into `out` (column-major,
     // N x f.spec_elems(FH), pitch rp(FH)).
@@ -730,6 +739,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
             if (!acc_fused) spec_acc.alloc(fft.spec_elems(s.FH));
             spec_xa.alloc(fa.spec_elems(c.H));
         }
+        // the spectra a column pass writes for a row C2R in the pair layout
+        // where both allow it (kWide: the summed correction only where the
+        // accumulating pass also inverts it; its separate TA inverse writes
+        // column-major spectra)
+        lay = fft.pair_ok() && (!kWide || !acc_fused || fa.row_pair_ok()) ? SL_PAIR : SL_COL;
         HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, npix, c.st));
         diag_cur.alloc(n);
         idiag_cur.alloc(1);
@@ -809,14 +823,15 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const size_t nout = c.nout();
         const size_t kstride = fft.spec_elems(c.H);
         const TF sc = scale(c);
+        const int l = iter_layout(c);
         auto mark = [&](int i) { if (ev) HIPCHK(hipEventRecord(ev[i], c.st)); };
         mark(0);
         HIPCHK(fft.row_r2c(ROW_PLAIN, x.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, c.H, 1, c.st));
         mark(1);
         HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, c.H, spec_k.p, kstride, c.H, psf_spec.p,
-                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st, l));
         mark(2);
-        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, sc, c.H, n, c.st));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, sc, c.H, n, c.st, l));
         mark(3);
         HIPCHK((gather_freeze<TF, TA>(d_ex.p, 0, n, max_nin, blur.p, nout, sci.p, w.p, frozen.p, wr.p, diag_cur.p,
                                       idiag_cur.p, c.st)));
@@ -837,7 +852,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
             HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
         }
         mark(m++);
-        if (!adjoint_sum(c, n, [&]() { mark(m++); }, true)) {
+        if (!adjoint_sum(c, n, [&]() { mark(m++); }, true, l)) {
             mark(m++);
             HIPCHK((update<TF, TA>(x.p, C.p, D.p, nout, (TA)max_sens, idiag_cur.p, partials.p, negs.p, ticket.p,
                                    c.st)));
@@ -849,6 +864,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
     }
+    int spectrum_layout(const Ctx &c) const override { return iter_layout(c); }
     void ensure_plain_buffers(Ctx &c) override {
         const size_t fstride = fft.spec_elems(fft.s.FH);
         if (!spec_tmp.p) spec_tmp.alloc(fstride);
@@ -1115,10 +1131,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
         xs.alloc(nout);
         from_host_f64(xin, nout, xs.p, c);
         const size_t kstride = fft.spec_elems(c.H);
+        const int l = iter_layout(c);
         HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, c.H, 1, c.st));
         HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, c.H, spec_k.p, kstride, c.H, psf_spec.p,
-                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
-        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st, l));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st, l));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(blur.p, (size_t)n * nout, bl, c);
     }
@@ -1130,10 +1147,11 @@ template <class TF, class TA> struct Pipe : PipeBase {
         xs.alloc(nout);
         from_host_f64(xin, nout, xs.p, c);
         const size_t kstride = fft.spec_elems(c.H);
+        const int l = iter_layout(c);
         HIPCHK(fft.row_r2c(ROW_PLAIN, xs.p, 0, c.W, c.W, nullptr, nullptr, 0, spec_x.p, 0, c.H, c.H, 1, c.st));
         HIPCHK(fft.template colpass<TA>(COL_BCAST, spec_x.p, 0, c.H, spec_k.p, kstride, c.H, psf_spec.p,
-                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st));
-        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st));
+                                        fft.spec_elems(s.FH), c.H, c.H, n, c.st, l));
+        HIPCHK(fft.row_c2r(spec_k.p, kstride, c.H, blur.p, nout, c.W, c.W, scale(c), c.H, n, c.st, l));
         HIPCHK(predict<TF>(d_ex.p, n, max_nin, blur.p, nout, wr.p, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(wr.p, c.npix(), pred, c);
@@ -1167,7 +1185,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const size_t kstride = fft.spec_elems(c.H);
         from_host_f64(v, c.npix(), wr.p, c);
         HIPCHK(fft.row_r2c(ROW_GATHER, nullptr, 0, 0, 0, d_ex.p, wr.p, c.W, spec_k.p, kstride, c.H, c.H, n, c.st));
-        adjoint_sum(c, n, []() {});
+        adjoint_sum(c, n, []() {}, false, iter_layout(c));
         HIPCHK(hipStreamSynchronize(c.st));
         to_host_f64(C.p, c.nout(), out, c);
     }
@@ -1476,13 +1494,14 @@ template <class T> static double time_iteration_transforms(Ctx &c, int FW, int F
         HIPCHK(hipMemsetAsync(ps.p, 0, (size_t)n * kf * sizeof(Cx<T>), c.st));
         HIPCHK(hipEventCreate(&a));
         HIPCHK(hipEventCreate(&b));
+        const int l = f.pair_ok() ? SL_PAIR : SL_COL;  // the layout the iteration takes at this shape
         auto trip = [&]() {
             HIPCHK(f.row_r2c(ROW_PLAIN, img.p, 0, c.W, c.W, nullptr, nullptr, 0, sx.p, 0, c.H, c.H, 1, c.st));
-            HIPCHK(f.template colpass<T>(COL_BCAST, sx.p, 0, c.H, sk.p, kh, c.H, ps.p, kf, c.H, c.H, n, c.st));
-            HIPCHK(f.row_c2r(sk.p, kh, c.H, img.p, nimg, c.W, c.W, (T)1, c.H, n, c.st));
+            HIPCHK(f.template colpass<T>(COL_BCAST, sx.p, 0, c.H, sk.p, kh, c.H, ps.p, kf, c.H, c.H, n, c.st, l));
+            HIPCHK(f.row_c2r(sk.p, kh, c.H, img.p, nimg, c.W, c.W, (T)1, c.H, n, c.st, l));
             HIPCHK(f.row_r2c(ROW_PLAIN, img.p, nimg, c.W, c.W, nullptr, nullptr, 0, sk.p, kh, c.H, c.H, n, c.st));
-            HIPCHK(f.template colpass<T>(COL_ADJ, sk.p, kh, c.H, sx.p, 0, c.H, ps.p, kf, c.H, c.H, n, c.st));
-            HIPCHK(f.row_c2r(sx.p, 0, c.H, img.p, 0, c.W, c.W, (T)1, c.H, 1, c.st));
+            HIPCHK(f.template colpass<T>(COL_ADJ, sk.p, kh, c.H, sx.p, 0, c.H, ps.p, kf, c.H, c.H, n, c.st, l));
+            HIPCHK(f.row_c2r(sx.p, 0, c.H, img.p, 0, c.W, c.W, (T)1, c.H, 1, c.st, l));
         };
         trip();
         for (int r = 0; r < 3; ++r) {
@@ -1671,7 +1690,7 @@ int updc_ctx_destroy(updc_ctx *ctx) {
 int updc_ctx_set_flags(updc_ctx *ctx, unsigned flags) {
     return guarded([&] {
         need(ctx != nullptr, UPDC_ERR_ARG, "null context");
-        const unsigned shape = UPDC_FLAG_PER_EXPOSURE | UPDC_FLAG_FUSED_ADJOINT;
+        const unsigned shape = UPDC_FLAG_PER_EXPOSURE | UPDC_FLAG_FUSED_ADJOINT | UPDC_FLAG_COLUMN_SPECTRA;
         if (((ctx->flags ^ flags) & shape) && ctx->pipe) ctx->pipe->invalidate_graph();
         if ((flags & UPDC_FLAG_PER_EXPOSURE) && ctx->prepared) {
             HIPCHK(hipSetDevice(ctx->device));
@@ -1842,6 +1861,14 @@ int updc_get_setup_cache_hits(updc_ctx *ctx, int *hits) {
     return guarded([&] {
         need(ctx != nullptr && hits, UPDC_ERR_ARG, "null argument");
         *hits = ctx->setup_cache_hits;
+    });
+}
+
+int updc_get_spectrum_layout(updc_ctx *ctx, int *layout) {
+    return guarded([&] {
+        need(ctx != nullptr && layout, UPDC_ERR_ARG, "null argument");
+        need(ctx->prepared, UPDC_ERR_STATE, "context not prepared");
+        *layout = ctx->pipe->spectrum_layout(*ctx);
     });
 }
 

@@ -14,7 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (FLAG_AUTOTUNE, FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FLAG_FUSED_ADJOINT, FreezeRow,
+from ._lib import (FLAG_AUTOTUNE, FLAG_GRAPH, FLAG_PER_EXPOSURE, FLAG_QUIET, FLAG_FUSED_ADJOINT, FLAG_COLUMN_SPECTRA,
+                   FreezeRow,
                    IterInfo, check)
 
 
@@ -132,6 +133,18 @@ class Context:
         or fused into the row R2C (False)."""
         self.flags = (self.flags & ~FLAG_FUSED_ADJOINT) if on else (self.flags | FLAG_FUSED_ADJOINT)
         check(self.lib.updc_ctx_set_flags(self.h, self.flags))
+
+    def set_column_spectra(self, on: bool) -> None:
+        """Keep the batched iteration's spectra column-major (True) instead of
+        the column-pair layout the FFT passes use where they can (False)."""
+        self.flags = (self.flags | FLAG_COLUMN_SPECTRA) if on else (self.flags & ~FLAG_COLUMN_SPECTRA)
+        check(self.lib.updc_ctx_set_flags(self.h, self.flags))
+
+    def spectrum_layout(self) -> str:
+        """'pairs' or 'columns': the layout of the batched iteration's spectra."""
+        v = C.c_int()
+        check(self.lib.updc_get_spectrum_layout(self.h, C.byref(v)))
+        return "pairs" if v.value == 1 else "columns"
 
     def set_graph(self, on: bool) -> None:
         self.flags = (self.flags | FLAG_GRAPH) if on else (self.flags & ~FLAG_GRAPH)
