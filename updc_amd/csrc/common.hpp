@@ -44,6 +44,20 @@ template <> struct OddRoots<7> {
 };
 #define UPDC_SQRT1_2 0.7071067811865475244008
 
+// ---------------------------------------------------------------- FFT arithmetic
+// The transforms' internal products use explicit fused multiply-adds (the
+// build contracts nothing by itself): a twiddle product in two multiplies and
+// two fmas instead of four multiplies and two adds, a butterfly's cosine and
+// sine terms as fmas.  Every transform kernel inlines these same helpers, so
+// the plain and batched forms still do identical arithmetic.  The spectrum
+// products of the iteration (PSF multiply, adjoint accumulation) keep the
+// reference's unfused form and do not use them.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <class T> __device__ __forceinline__ Cx<T> cmul_f(Cx<T> a, Cx<T> w) {
+    return cx<T>(fma_t(a.re, w.re, -(a.im * w.im)), fma_t(a.re, w.im, a.im * w.re));
+}
+
 // ---------------------------------------------------------------- butterflies
 // In-place DFT of length R on v[0..R): y_j = sum_q v_q exp(-+2 pi i j q / R)
 // (minus for forward, plus for inverse), unnormalised.
@@ -96,8 +110,8 @@ template <int P, bool INV, class T> __device__ __forceinline__ void bfly_odd(Cx<
             const int jm = (j * m) % P;
             const T c = (T)OddRoots<P>::c[jm <= H ? jm : P - jm];
             const T s = (jm <= H) ? (T)OddRoots<P>::s[jm] : -(T)OddRoots<P>::s[P - jm];
-            re = re + cscale(a[m], c);
-            im = im + cscale(b[m], s);
+            re = cx<T>(fma_t(a[m].re, c, re.re), fma_t(a[m].im, c, re.im));
+            im = cx<T>(fma_t(b[m].re, s, im.re), fma_t(b[m].im, s, im.im));
         }
         // forward: y_j = re - i*im, y_{P-j} = re + i*im
         Cx<T> iim = cx<T>(-im.im, im.re);
@@ -143,7 +157,7 @@ template <bool INV, class T> __device__ __forceinline__ void bfly16(Cx<T> *v) {
 #pragma unroll
         for (int k1 = 1; k1 < 4; ++k1) {
             const int e = n2 * k1;
-            y[n2][k1] = cmul(y[n2][k1], cx<T>((T)c[e], INV ? (T)s[e] : -(T)s[e]));
+            y[n2][k1] = cmul_f(y[n2][k1], cx<T>((T)c[e], INV ? (T)s[e] : -(T)s[e]));
         }
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) {
@@ -230,7 +244,7 @@ __device__ __forceinline__ void fft_stage(Cx<T> *buf, int L, int lg, int cfix, i
                 for (int q = 1; q < R; ++q) {
                     Cx<T> w = wk[q - 1];
                     if (INV) w.im = -w.im;
-                    v[r][q] = cmul(v[r][q], w);
+                    v[r][q] = cmul_f(v[r][q], w);
                 }
             }
             bfly<R, INV>(v[r]);
@@ -289,7 +303,7 @@ __device__ __forceinline__ void fft_stage_pp(const Cx<T> *__restrict__ in, Cx<T>
                 for (int q = 1; q < R; ++q) {
                     Cx<T> w = wk[q - 1];
                     if (INV) w.im = -w.im;
-                    v[q] = cmul(v[q], w);
+                    v[q] = cmul_f(v[q], w);
                 }
             }
             bfly<R, INV>(v);
@@ -368,7 +382,7 @@ __device__ __forceinline__ void edge_last(Cx<T> (&rb)[NBE][RE], const Cx<T> *buf
             for (int q = 1; q < RE; ++q) {
                 Cx<T> w = wk[q - 1];
                 if (INV) w.im = -w.im;
-                v[q] = cmul(v[q], w);
+                v[q] = cmul_f(v[q], w);
             }
             bfly<RE, INV>(v);
 #pragma unroll

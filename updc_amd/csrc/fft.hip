@@ -699,7 +699,13 @@ __global__ __launch_bounds__(NT) void k_col(FftLen f, int nc, const Cx<T> *__res
 // their barriers fewer per transform than the LDS-only kernel.  The
 // butterflies, twiddles and their order are those of the LDS-only kernel.
 // PF: the next PSF column (BCAST) or the current one (ADJ) is loaded into
-// registers before the transform that precedes its use.
+// registers before the transform that precedes its use.  The accumulate pass
+// (COL_ACC) loads its PSF column after the transform: held through it, the
+// column's registers and the FP64 sum made the pass spill once the
+// transforms' products became fused multiply-adds (0.66 -> 0.83 ms at 5600
+// rows; issued between the middle stages and the last edge stage, it still
+// spills), and without them it runs faster than with the prefetch before
+// (0.625 ms).
 // EV: complex values per thread per register set (the edge butterflies of
 // one thread, EV / RE of them).
 // PP: the LDS holds two transform buffers (2 FH values).  The middle stages
@@ -1289,7 +1295,7 @@ template <class T, class TA, int MODE, bool PF, bool PP, bool CP> static void *c
 // pp: the two-buffer kernel (FP32 only; FP64 columns fill the LDS with one).
 // cp: SL_PAIR in and out (the batched modes only; nullptr for COL_FWD/COL_INV)
 template <class T, class TA, int MODE> static void *col_e_fn(int re, bool pp, bool cp = false) {
-    constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ || MODE == COL_ACC);
+    constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ);  // not COL_ACC (k_col_e)
     if (cp) {
         if constexpr (MODE == COL_FWD || MODE == COL_INV) return nullptr;
         if constexpr (sizeof(T) == 4)
