@@ -1452,6 +1452,9 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
             return "cannot query the device";
         row_grid = std::max(8, prop.multiProcessorCount / 8 * 8);
         constexpr int RB = RowCfg<T>::RB, NT = RowCfg<T>::NT;
+        // the R2C keeps its twiddle tables in global memory, so it fits rows
+        // the C2R (tables in LDS, a larger prefetch) does not
+        r2c_persistent = row_persistent && !long_row && s.M <= RowCfg<T>::PFR * NT && (size_t)rlds <= LDS_BUDGET;
         row_persistent = row_persistent && !long_row && (s.M + 1) * RB <= RowCfg<T>::PFM * NT &&
                          s.M <= RowCfg<T>::PFR * NT && (size_t)rlds_p <= LDS_BUDGET;
     }
@@ -1525,7 +1528,7 @@ hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pit
     const size_t lds = row_lds<T>(s.M);
     dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
     const int vec = pairs_aligned<T>(in, in_pitch, in_bstride);
-    if (mode == ROW_PLAIN && row_persistent) {
+    if (mode == ROW_PLAIN && r2c_persistent) {
         hipLaunchKernelGGL((k_row_r2c_p<T, NT, VMAX, RB, RowCfg<T>::PFR>), dim3(row_grid), dim3(NT), lds, st, row, post,
                            in, in_bstride, in_pitch, win, out, out_bstride, rp(rp_rows), rows, batch, vec);
         return hipGetLastError();

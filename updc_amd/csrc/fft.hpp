@@ -87,7 +87,9 @@ template <class T> struct Fft2 {
     FftLen col_wide{};
     void *tw_col_wide = nullptr;
     bool allow_edge = true;  // set before init(): false forces the LDS-only column kernel
-    bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit
+    bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit;
+                                 // after init(): the persistent row C2R passes run
+    bool r2c_persistent = false;  // after init(): the persistent row R2C runs (it fits longer rows than the C2R)
     bool row_odd_first = true;   // set before init(): an odd radix first in the row plan (LDS banks)
     int row_grid = 0;            // workgroups of the persistent row passes (one per CU)
     int row_ntw = 0;             // entries of the row plan's stage twiddle tables
