@@ -479,6 +479,35 @@ def test_column_pair_layout_matches_column_major(small_ds):
             assert out[0][1] == out[1][1], (shape, prec)
 
 
+def test_long_row_c2r_matches_oracle(small_ds):
+    """FFT 7000 x 200: rows of 3500 points are too long for the 4-row
+    persistent C2R, so the FP32 tiers take its 2-row form (and the R2C stays
+    persistent); every tier against the FP64 oracle within its usual bound,
+    and the column-pair layout against column-major bit for bit (columns of
+    200 take the edge column kernel, so the FP32 tiers write pairs)."""
+    from oracle import oracle as O
+    shape = (7000, 200)
+    print(f"case: FFT {shape}, 2-row persistent C2R", flush=True)
+    ref = O.RefSolver(small_ds.W, small_ds.H, oracle_exposures(small_ds), "f64", fft_shape=shape)
+    ref.iterate(3)
+    for prec, tol in (("f64", 1e-11), ("mixed", 1e-4), ("f32", 1e-4)):
+        xs = []
+        for column in (False, True):
+            c = _ctx(small_ds, prec, fft=shape, graph=True)
+            c.set_column_spectra(column)
+            c.prepare()
+            # FP64 rows of 3500 points take the one-tile kernels and column-major spectra
+            want = "columns" if column or prec == "f64" else "pairs"
+            assert c.spectrum_layout() == want, (prec, column)
+            c.iterate(3)
+            xs.append(c.estimate(f64=True))
+            c.close()
+        st = O.compare_stats(ref.x, xs[0])
+        print(f"  {prec}: norm_linf {st['norm_linf']:.3e}", flush=True)
+        assert st["norm_linf"] < tol, (prec, st)
+        assert np.array_equal(xs[0], xs[1]), prec
+
+
 def test_radix10_column_plan_matches_oracle(small_ds):
     """FFT 112 x 200: columns of FH = 200 = 2^3 5^2 take radix-10 edge stages
     (10 2 10, each a prime-factor 2 x 5 butterfly) in the FP32 column passes,

@@ -1096,6 +1096,10 @@ template <class T> struct RowCfg;
 // them (FP32: M > 3071) use the one-tile-per-workgroup kernels.
 template <> struct RowCfg<float> { static constexpr int NT = 1024, VMAX = 16, RB = 4, PFM = 12, PFR = 4; };
 template <> struct RowCfg<double> { static constexpr int NT = 1024, VMAX = 10, RB = 2, PFM = 6, PFR = 3; };
+// FP32 rows too long for the 4-row persistent C2R (M > 3071) take 2-row
+// tiles: 8 prefetch registers cover M <= 4095, and the tile with the LDS
+// tables fits (123 KB at M = 3840).
+struct RowCfgL { static constexpr int NT = 1024, VMAX = 16, RB = 2, PFM = 8; };
 template <class T> struct ColCfg;
 // PF: PSF prefetch in the edge column kernel (FP64 would spill with it).
 // EV: complex values per thread of the edge kernel's register sets.
@@ -1251,6 +1255,9 @@ template <class T> static int pairs_aligned(const void *p, int pitch, size_t bst
 template <class T, bool CP> static void *row_c2r_p_fn() {
     return (void *)k_row_c2r_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM, CP>;
 }
+template <bool CP> static void *row_c2r_pl_fn() {
+    return (void *)k_row_c2r_p<float, RowCfgL::NT, RowCfgL::VMAX, RowCfgL::RB, RowCfgL::PFM, CP>;
+}
 template <class T, class TF, bool CP> static void *row_c2r_upd_fn() {
     return (void *)k_row_c2r_upd_p<T, TF, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM, CP>;
 }
@@ -1267,6 +1274,10 @@ template <class T> static size_t row_lds(int M) { return (size_t)RowCfg<T>::RB *
 // (FP32) the split roots
 template <class T> static size_t row_lds_p(int M, int ntw) {
     return row_lds<T>(M) + ((size_t)ntw + (RowPL<T>::value ? (size_t)M + 1 : 0)) * sizeof(Cx<T>);
+}
+// ... and of the FP32 2-row persistent C2R
+static size_t row_lds_pl(int M, int ntw) {
+    return (size_t)RowCfgL::RB * (M + 1) * sizeof(Cx<float>) + ((size_t)ntw + M + 1) * sizeof(Cx<float>);
 }
 // entries of a plan's stage twiddle tables (as upload_stage_twiddles lays them out)
 static int stage_twiddle_count(const FftLen &f) {
@@ -1455,8 +1466,18 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
         // the R2C keeps its twiddle tables in global memory, so it fits rows
         // the C2R (tables in LDS, a larger prefetch) does not
         r2c_persistent = row_persistent && !long_row && s.M <= RowCfg<T>::PFR * NT && (size_t)rlds <= LDS_BUDGET;
+        const bool allow = row_persistent;
         row_persistent = row_persistent && !long_row && (s.M + 1) * RB <= RowCfg<T>::PFM * NT &&
                          s.M <= RowCfg<T>::PFR * NT && (size_t)rlds_p <= LDS_BUDGET;
+        if constexpr (sizeof(T) == 4) {
+            c2r_long = allow && !long_row && !row_persistent && (s.M + 1) * RowCfgL::RB <= RowCfgL::PFM * NT &&
+                       row_lds_pl(s.M, row_ntw) <= LDS_BUDGET;
+            if (c2r_long)
+                for (void *fn : {row_c2r_pl_fn<false>(), row_c2r_pl_fn<true>()})
+                    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)row_lds_pl(s.M, row_ntw)) != hipSuccess)
+                        return "cannot raise LDS limit for the row pass";
+        }
     }
     if (!long_row)
         for (void *fn : fns_row)
@@ -1561,6 +1582,20 @@ hipError_t Fft2<T>::row_c2r(const Cx<T> *in, size_t in_bstride, int rp_rows, T *
                                row_lds_p<T>(s.M, row_ntw), st, row, post, row_ntw, in, in_bstride, rp(rp_rows), out,
                                out_bstride, out_pitch, wout, scale, rows, batch, vec);
         return hipGetLastError();
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (c2r_long) {
+            using L = RowCfgL;
+            if (lin == SL_PAIR)
+                hipLaunchKernelGGL((k_row_c2r_p<T, L::NT, L::VMAX, L::RB, L::PFM, true>), dim3(row_grid), dim3(L::NT),
+                                   row_lds_pl(s.M, row_ntw), st, row, post, row_ntw, in, in_bstride, rp(rp_rows), out,
+                                   out_bstride, out_pitch, wout, scale, rows, batch, vec);
+            else
+                hipLaunchKernelGGL((k_row_c2r_p<T, L::NT, L::VMAX, L::RB, L::PFM, false>), dim3(row_grid),
+                                   dim3(L::NT), row_lds_pl(s.M, row_ntw), st, row, post, row_ntw, in, in_bstride,
+                                   rp(rp_rows), out, out_bstride, out_pitch, wout, scale, rows, batch, vec);
+            return hipGetLastError();
+        }
     }
     const dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
     hipLaunchKernelGGL((k_row_c2r<T, NT, VMAX, RB>), grid, dim3(NT), lds, st, row, post,
@@ -1683,7 +1718,12 @@ template <class T> bool Fft2<T>::row_pair_ok() const {
     // the C2R prefetch slots must cover the tile's pairs: (M + 2) / 2 pairs of
     // RB rows, pair_lpp lanes each, pair_cpl complex values per lane
     const long slots = (long)(RowCfg<T>::PFM / pair_cpl<T>()) * NT;
-    return row_persistent && !long_row && (long)((s.M + 2) / 2) * RB * pair_lpp<T>() <= slots;
+    if (row_persistent) return !long_row && (long)((s.M + 2) / 2) * RB * pair_lpp<T>() <= slots;
+    if constexpr (sizeof(T) == 4) {
+        const long slots_l = (long)(RowCfgL::PFM / pair_cpl<T>()) * RowCfgL::NT;
+        if (c2r_long) return (long)((s.M + 2) / 2) * RowCfgL::RB * pair_lpp<T>() <= slots_l;
+    }
+    return false;
 }
 
 template <class T> bool Fft2<T>::acc_inverse_ok(const FftLen &fa) const {

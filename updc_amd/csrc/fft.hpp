@@ -90,6 +90,8 @@ template <class T> struct Fft2 {
     bool row_persistent = true;  // set before init(): persistent pipelined row passes where they fit;
                                  // after init(): the persistent row C2R passes run
     bool r2c_persistent = false;  // after init(): the persistent row R2C runs (it fits longer rows than the C2R)
+    bool c2r_long = false;        // after init(): FP32 rows too long for the 4-row persistent C2R take a 2-row one
+                                  // (row_c2r only; row_c2r_update needs row_persistent)
     bool row_odd_first = true;   // set before init(): an odd radix first in the row plan (LDS banks)
     int row_grid = 0;            // workgroups of the persistent row passes (one per CU)
     int row_ntw = 0;             // entries of the row plan's stage twiddle tables
@@ -119,9 +121,9 @@ template <class T> struct Fft2 {
     // phantom partner of the last column when NC is odd)
     size_t spec_elems(int rows) const { return (size_t)(s.NC + (s.NC & 1)) * rp(rows); }
     // Whether the batched passes accept SL_PAIR spectra: the row C2R passes
-    // (row_c2r, row_c2r_update) where they are persistent and no row is
-    // four-step, the column passes' outputs (colpass, colacc) where the edge
-    // column kernel runs.
+    // (row_c2r, row_c2r_update) where they are persistent (row_c2r also in
+    // its 2-row form) and no row is four-step, the column passes' outputs
+    // (colpass, colacc) where the edge column kernel runs.
     bool row_pair_ok() const;
     bool col_pair_ok() const { return col_re > 0 && !long_col; }
     bool pair_ok() const { return row_pair_ok() && col_pair_ok(); }

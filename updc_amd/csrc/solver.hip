@@ -351,6 +351,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
     bool acc_fused = false;   // kWide: that inverse runs inside the accumulating column pass
     bool fused_update = false;  // the final row C2R applies the update (persistent row pass available)
     int lay = SL_COL;           // layout of the spectra the batched iteration's column passes write
+    bool corr_pair = false;     // kWide: the summed correction also leaves its column pass in lay
     bool any_c16 = false, any_wide = false;  // exposures with / without the compact CSC
     DevBuf<ExpDiag> diag_cur, diag_hist;
     DevBuf<IterDiag> idiag_cur, idiag_hist;
@@ -383,7 +384,7 @@ template <class TF, class TA> struct Pipe : PipeBase {
         const size_t kstride = fft.spec_elems(c.H), fstride = fft.spec_elems(s.FH);
         if constexpr (kWide) {
             // the sum leaves the accumulating pass in layout la
-            const int la = acc_fused ? l : SL_COL;
+            const int la = acc_fused && corr_pair ? l : SL_COL;
             if (acc_fused) {
                 // the column inverse of the sum runs in TA inside the accumulating pass
                 HIPCHK(fft.template colacc<TA>(spec_k.p, kstride, c.H, spec_xa.p, c.H, c.H, psf_spec.p, fstride, c.H,
@@ -741,9 +742,10 @@ template <class TF, class TA> struct Pipe : PipeBase {
         }
         // the spectra a column pass writes for a row C2R in the pair layout
         // where both allow it (kWide: the summed correction only where the
-        // accumulating pass also inverts it; its separate TA inverse writes
-        // column-major spectra)
-        lay = fft.pair_ok() && (!kWide || !acc_fused || fa.row_pair_ok()) ? SL_PAIR : SL_COL;
+        // accumulating pass also inverts it and the TA row C2R takes pairs;
+        // a separate TA inverse writes column-major spectra)
+        lay = fft.pair_ok() ? SL_PAIR : SL_COL;
+        corr_pair = kWide && acc_fused && fa.row_pair_ok();
         HIPCHK(fill<uint8_t>(frozen.p, (uint8_t)0, npix, c.st));
         diag_cur.alloc(n);
         idiag_cur.alloc(1);
