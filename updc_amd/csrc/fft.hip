@@ -1304,7 +1304,7 @@ template <class T, class TA, int MODE, bool PF, bool PP, bool CP> static void *c
     }
 }
 // pp: the two-buffer kernel (FP32 only; FP64 columns fill the LDS with one).
-// cp: SL_PAIR in and out (the batched modes only; nullptr for COL_FWD/COL_INV)
+// cp: the output in SL_PAIR (the batched modes only; nullptr for COL_FWD/COL_INV)
 template <class T, class TA, int MODE> static void *col_e_fn(int re, bool pp, bool cp = false) {
     constexpr bool PF = ColCfg<T>::PF && (MODE == COL_BCAST || MODE == COL_ADJ);  // not COL_ACC (k_col_e)
     if (cp) {
