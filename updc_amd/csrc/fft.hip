@@ -431,10 +431,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
     // the detector stage of this iteration, or an earlier iteration, failed:
     // the reference never reaches this update, so x stays as it is
     const bool hold = idiag->halt || (idiag->err & (UPDC_E_MU | UPDC_E_RATIO));
-    auto upd = [&](size_t t, T c) __attribute__((always_inline)) {
-        if (hold) return;
-        const T d = D[t];
-        TF xv = x[t];
+    auto upd = [&](size_t t, T c, T d, TF xv) __attribute__((always_inline)) {
         unsigned int b = 0;
         if (d > thr) {
             T fct = c / d;
@@ -458,16 +455,24 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
         sum += (double)xv;
         nneg += xv < (TF)0 ? 1 : 0;
     };
+    // The D and x values of a column step are loaded before its updates: one
+    // update's store would otherwise hold up the next element's loads (a
+    // wave's vector memory counter drains in issue order).
     auto emit = [&](int, int r0, const Cx<T> *zz) __attribute__((always_inline)) {
+        if (hold) return;
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
             if (r0 + j >= rows) continue;
             const size_t rowt = (size_t)(r0 + j) * W;
             for (int n = threadIdx.x; n < M; n += NT) {
                 const int c = 2 * n;
+                // clamped indexes: every load in bounds, unused values discarded
+                const size_t t0 = rowt + min(c, W - 1), t1 = rowt + min(c + 1, W - 1);
+                const T d0 = D[t0], d1 = D[t1];
+                const TF x0 = x[t0], x1 = x[t1];
                 const Cx<T> v = zz[(n << LG) + j];
-                if (c < W) upd(rowt + c, v.re * scale);
-                if (c + 1 < W) upd(rowt + c + 1, v.im * scale);
+                if (c < W) upd(rowt + c, v.re * scale, d0, x0);
+                if (c + 1 < W) upd(rowt + c + 1, v.im * scale, d1, x1);
             }
         }
     };
