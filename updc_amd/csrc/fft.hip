@@ -464,15 +464,26 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
         for (int j = 0; j < RB; ++j) {
             if (r0 + j >= rows) continue;
             const size_t rowt = (size_t)(r0 + j) * W;
-            for (int n = threadIdx.x; n < M; n += NT) {
-                const int c = 2 * n;
-                // clamped indexes: every load in bounds, unused values discarded
-                const size_t t0 = rowt + min(c, W - 1), t1 = rowt + min(c + 1, W - 1);
-                const T d0 = D[t0], d1 = D[t1];
-                const TF x0 = x[t0], x1 = x[t1];
-                const Cx<T> v = zz[(n << LG) + j];
-                if (c < W) upd(rowt + c, v.re * scale, d0, x0);
-                if (c + 1 < W) upd(rowt + c + 1, v.im * scale, d1, x1);
+            for (int n0 = threadIdx.x; n0 < M; n0 += 2 * NT) {
+                T dv[2][2];
+                TF xv[2][2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        // clamped index: every load in bounds, unused values discarded
+                        const size_t tt = rowt + min(2 * (n0 + u * NT) + h, W - 1);
+                        dv[u][h] = D[tt];
+                        xv[u][h] = x[tt];
+                    }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int n = n0 + u * NT, c = 2 * n;
+                    if (n >= M) break;
+                    const Cx<T> v = zz[(n << LG) + j];
+                    if (c < W) upd(rowt + c, v.re * scale, dv[u][0], xv[u][0]);
+                    if (c + 1 < W) upd(rowt + c + 1, v.im * scale, dv[u][1], xv[u][1]);
+                }
             }
         }
     };
