@@ -337,6 +337,11 @@ __device__ __forceinline__ void row_c2r_tiles(const FftLen &f, const Cx<T> *twl,
         const int b = t / nblk, r0 = (t - b * nblk) * RB;
         const bool more = t + step < t_end;
         __syncthreads();  // the previous tile's LDS reads are done
+        // The prefetched tile has landed (vmcnt(0)) on every path.  The
+        // compiler's own waits below sit in lane-masked branches that a wave
+        // may skip, after which it would wait again before reusing a
+        // register: later, with the next tile's first third in flight.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         if constexpr (CP) {
 #pragma unroll
             for (int m = 0; m < NS; ++m) {
