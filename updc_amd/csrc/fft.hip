@@ -547,11 +547,13 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_c2r_upd_p(
 }
 
 // R2C of real rows (as k_row_r2c<..., ROW_PLAIN>).  PFR (x[2n], x[2n+1]) pairs
-// per thread and row cover M pairs.
-template <class T, int NT, int VMAX, int RB, int PFR>
+// per thread and row cover M pairs.  VEC: rows whose pairs are 8-byte
+// aligned (pairs_aligned), read with one load per pair; a kernel that also
+// held the unaligned form's code spilled (FP32, 12 bytes).
+template <class T, int NT, int VMAX, int RB, int PFR, bool VEC>
 __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c_p(
     FftLen f, const Cx<T> *__restrict__ post, const T *__restrict__ in, size_t in_bstride, int in_pitch, int win,
-    Cx<T> *__restrict__ out, size_t out_bstride, int RP, int rows, int batch, int vec) {
+    Cx<T> *__restrict__ out, size_t out_bstride, int RP, int rows, int batch) {
     constexpr int LG = RB == 1 ? 0 : RB == 2 ? 1 : RB == 4 ? 2 : 3;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cx<T> *z = reinterpret_cast<Cx<T> *>(smem_raw);
@@ -569,7 +571,7 @@ __global__ __launch_bounds__(NT, (row_min_waves<T, NT>())) void k_row_r2c_p(
             for (int m = 0; m < PFR; ++m) {
                 const int n = threadIdx.x + m * NT, c = 2 * n;
                 const bool ok = n < M && r < rows;
-                pre[j][m] = ok ? load_pair(row, c, win, vec != 0) : cx<T>((T)0, (T)0);
+                pre[j][m] = ok ? load_pair(row, c, win, VEC) : cx<T>((T)0, (T)0);
             }
         }
     };
@@ -1282,8 +1284,8 @@ template <bool CP> static void *row_c2r_pl_fn() {
 template <class T, class TF, bool CP> static void *row_c2r_upd_fn() {
     return (void *)k_row_c2r_upd_p<T, TF, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFM, CP>;
 }
-template <class T> static void *row_r2c_p_fn() {
-    return (void *)k_row_r2c_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFR>;
+template <class T, bool VEC> static void *row_r2c_p_fn() {
+    return (void *)k_row_r2c_p<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB, RowCfg<T>::PFR, VEC>;
 }
 template <class T> static void *row_c2r_fn() {
     return (void *)k_row_c2r<T, RowCfg<T>::NT, RowCfg<T>::VMAX, RowCfg<T>::RB>;
@@ -1473,7 +1475,8 @@ template <class T> std::string Fft2<T>::init(int FW, int FH) {
     // kernel's out-of-place stages)
     col_pp = allow_pp && col_re && sizeof(T) == 4 && col_pp_lds<T>(FH, col) <= LDS_BUDGET;
     const int col_lds = (int)(col_pp ? col_pp_lds<T>(FH, col) : (size_t)FH * sizeof(Cx<T>));
-    void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(), row_r2c_p_fn<T>()};
+    void *fns_row[] = {row_r2c_fn<T, ROW_PLAIN>(), row_r2c_fn<T, ROW_GATHER>(), row_c2r_fn<T>(),
+                       row_r2c_p_fn<T, false>(), row_r2c_p_fn<T, true>()};
     void *fns_row_p[] = {row_c2r_p_fn<T, false>(), row_c2r_upd_fn<T, float, false>(), row_c2r_upd_fn<T, T, false>(),
                          row_c2r_p_fn<T, true>(), row_c2r_upd_fn<T, float, true>(), row_c2r_upd_fn<T, T, true>()};
     // persistent row passes: one workgroup per CU, prefetch registers must cover a tile
@@ -1571,8 +1574,12 @@ hipError_t Fft2<T>::row_r2c(int mode, const T *in, size_t in_bstride, int in_pit
     dim3 grid(((rows + RB - 1) / RB + 7) / 8 * 8, batch);  // a multiple of 8 for xcd_tile
     const int vec = pairs_aligned<T>(in, in_pitch, in_bstride);
     if (mode == ROW_PLAIN && r2c_persistent) {
-        hipLaunchKernelGGL((k_row_r2c_p<T, NT, VMAX, RB, RowCfg<T>::PFR>), dim3(row_grid), dim3(NT), lds, st, row, post,
-                           in, in_bstride, in_pitch, win, out, out_bstride, rp(rp_rows), rows, batch, vec);
+        if (vec)
+            hipLaunchKernelGGL((k_row_r2c_p<T, NT, VMAX, RB, RowCfg<T>::PFR, true>), dim3(row_grid), dim3(NT), lds, st,
+                               row, post, in, in_bstride, in_pitch, win, out, out_bstride, rp(rp_rows), rows, batch);
+        else
+            hipLaunchKernelGGL((k_row_r2c_p<T, NT, VMAX, RB, RowCfg<T>::PFR, false>), dim3(row_grid), dim3(NT), lds,
+                               st, row, post, in, in_bstride, in_pitch, win, out, out_bstride, rp(rp_rows), rows, batch);
         return hipGetLastError();
     }
     if (mode == ROW_PLAIN)
